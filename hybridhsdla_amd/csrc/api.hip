@@ -1,0 +1,888 @@
+// C-ABI implementation of hsgen (include/hsgen.h): contexts, device-resident
+// systems, and the per-k-point pipeline
+//   abgen (A/B generator) -> tmul (W_a = M_a Z_a) -> lower-triangle S and H.
+//
+// Operand layout in HBM, per k-point (complex128, column-major, ld = ldz):
+//   Z  = [A_1; B_1; A_2; B_2; ...]   rows 2*off_a .. 2*off_a + 2*nlm_a - 1
+//   Zs = [A_1; N_1 B_1; ...]         N_a = diag(||u_dot_l||^2)
+//   W  = [M_1 Z_1; M_2 Z_2; ...]     M_a = [[T_aa, T_ab], [T_ab^H, T_bb]]
+// so that   S = Z^H Zs   and   H = Z^H W   (only i >= j stored).
+// ldz = roundup(2K, BK) + BK, columns padded to the tile size; every padding
+// element is zero so the kernels need no bounds checks along k.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hsgen.h"
+#include "hsgen_internal.h"
+
+using namespace hsgen;
+
+static const int kDefaultTile = 1;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct hsgen_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    std::string err;
+    // workspace
+    DevBuf Z, Zs, W, Htmp, Stmp, A, B, g, probs, work, unorm2, atoms_tmp;
+};
+
+struct hsgen_sys {
+    int natoms = 0, max_lmax = 0, nl = 0, L = 0, kdim = 0;
+    double omega = 0.0, vol_prefac = 0.0;
+    double bmat[9];
+    std::vector<AtomDev> atoms;        // host copy
+    std::vector<int64_t> m_off;        // per atom: offset of M_a in d_M (complex units)
+    std::vector<int32_t> m_ld;         // per atom: leading dim of M_a
+    std::vector<int32_t> m_cols;       // per atom: padded column count
+    int m_bk = 0, m_bm = 0;            // tile geometry M was padded for
+    int64_t sum_nlm2 = 0;              // sum_a nlm_a^2
+    AtomDev* d_atoms = nullptr;
+    RadDev* d_rad = nullptr;
+    double* d_ynorm = nullptr;
+    double2* d_M = nullptr;
+};
+
+namespace {
+
+hsgen_status fail(hsgen_ctx* ctx, hsgen_status s, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return s;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ctx, HSGEN_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+hsgen_status ensure(hsgen_ctx* ctx, DevBuf& b, size_t bytes, bool zero_new)
+{
+    if (b.bytes >= bytes && b.p) return HSGEN_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return fail(ctx, HSGEN_ERR_HIP, "hipFree: %s", hipGetErrorString(e));
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, HSGEN_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    if (zero_new) {
+        e = hipMemsetAsync(b.p, 0, bytes, ctx->stream);
+        if (e != hipSuccess) return fail(ctx, HSGEN_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    }
+    return HSGEN_OK;
+}
+
+void release(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+bool is_device_ptr(const void* p)
+{
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+inline int64_t roundup(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// ylm4 normalisation table (ylm4.f:236-262), index l(l+1)+m.
+std::vector<double> ynorm_table(int lmax)
+{
+    const int L = (lmax + 1) * (lmax + 1);
+    std::vector<double> yn(L);
+    const double fpi = 4.0 * kPi;
+    for (int l = 0; l <= lmax; ++l) {
+        const int lm0 = l * (l + 1);
+        double cd = 1.0;
+        const double a = std::sqrt((2 * l + 1) / fpi);
+        yn[lm0] = a;
+        for (int m = 1; m <= l; ++m) {
+            cd = cd / ((double)(l + 1 - m) * (double)(l + m));
+            yn[lm0 + m] = a * std::sqrt(cd);
+            yn[lm0 - m] = ((m & 1) ? -1.0 : 1.0) * yn[lm0 + m];
+        }
+    }
+    return yn;
+}
+
+struct Geometry {
+    int cfg;
+    ZgemmLaunchCfg t;
+    int64_t kpad;   // kdepth of the S/H contraction = roundup(2K, BK)
+    int64_t ldz;    // kpad + BK
+    int64_t gp;     // padded column count
+};
+
+Geometry geometry(int tile, int kdim, int ng)
+{
+    Geometry g;
+    g.cfg = tile > 0 ? tile : kDefaultTile;
+    g.t = zgemm_cfg(g.cfg);
+    if (g.t.bm == 0) {
+        g.cfg = kDefaultTile;
+        g.t = zgemm_cfg(g.cfg);
+    }
+    const int64_t tm = std::max(g.t.bm, g.t.bn);
+    g.kpad = roundup(2 * (int64_t)kdim, g.t.bk);
+    g.ldz = g.kpad + g.t.bk;
+    g.gp = roundup(std::max(ng, 1), tm);
+    return g;
+}
+
+// Build the per-atom M_a = [[T_aa, T_ab], [T_ab^H, T_bb]] (T_aa, T_bb taken
+// from their lower triangles) padded for tile geometry (bk, bm), deduplicated
+// by T pointer triple, and upload.
+hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const double* const* taa,
+                      const double* const* tab, const double* const* tbb, int bk, int bm)
+{
+    if (s->d_M) {
+        (void)hipFree(s->d_M);
+        s->d_M = nullptr;
+    }
+    const int na = s->natoms;
+    s->m_off.assign(na, 0);
+    s->m_ld.assign(na, 0);
+    s->m_cols.assign(na, 0);
+    std::map<std::vector<const void*>, int> seen;
+    std::vector<int> owner(na, -1);
+    int64_t total = 0;
+    for (int a = 0; a < na; ++a) {
+        std::vector<const void*> key{taa[a], tab[a], tbb[a], (const void*)(intptr_t)nlm[a]};
+        auto it = seen.find(key);
+        const int n2 = 2 * nlm[a];
+        s->m_ld[a] = (int32_t)roundup(n2, bk);
+        s->m_cols[a] = (int32_t)roundup(n2, bm);
+        if (it != seen.end()) {
+            owner[a] = it->second;
+            s->m_off[a] = s->m_off[it->second];
+            continue;
+        }
+        seen[key] = a;
+        owner[a] = a;
+        s->m_off[a] = total;
+        total += (int64_t)s->m_ld[a] * s->m_cols[a];
+    }
+    std::vector<std::complex<double>> hM((size_t)std::max<int64_t>(total, 1), 0.0);
+    for (int a = 0; a < na; ++a) {
+        if (owner[a] != a) continue;
+        const int n = nlm[a];
+        const int64_t ld = s->m_ld[a];
+        std::complex<double>* M = hM.data() + s->m_off[a];
+        const std::complex<double>* Taa = (const std::complex<double>*)taa[a];
+        const std::complex<double>* Tab = (const std::complex<double>*)tab[a];
+        const std::complex<double>* Tbb = (const std::complex<double>*)tbb[a];
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                // Hermitian expansion from the lower triangle
+                std::complex<double> aa = r >= c ? Taa[r + (int64_t)n * c] : std::conj(Taa[c + (int64_t)n * r]);
+                std::complex<double> bb = r >= c ? Tbb[r + (int64_t)n * c] : std::conj(Tbb[c + (int64_t)n * r]);
+                if (r == c) { aa = aa.real(); bb = bb.real(); }
+                M[r + ld * c] = aa;
+                M[(n + r) + ld * (n + c)] = bb;
+                M[r + ld * (n + c)] = Tab[r + (int64_t)n * c];              // T_ab block
+                M[(n + r) + ld * c] = std::conj(Tab[c + (int64_t)n * r]);   // T_ab^H block
+            }
+    }
+    HIPCHK(hipMalloc(&s->d_M, sizeof(double2) * hM.size()));
+    HIPCHK(hipMemcpy(s->d_M, hM.data(), sizeof(double2) * hM.size(), hipMemcpyHostToDevice));
+    s->m_bk = bk;
+    s->m_bm = bm;
+    return HSGEN_OK;
+}
+
+// Keep the copy of T pointers so M can be rebuilt for another tile geometry.
+struct SysExtra {
+    std::vector<std::vector<std::complex<double>>> taa, tab, tbb;
+};
+std::map<const hsgen_sys*, SysExtra> g_extra;
+
+hsgen_status ensure_M(hsgen_ctx* ctx, hsgen_sys* s, int bk, int bm)
+{
+    if (s->d_M && s->m_bk == bk && s->m_bm == bm) return HSGEN_OK;
+    SysExtra& x = g_extra[s];
+    const int na = s->natoms;
+    std::vector<const double*> pa(na), pb(na), pc(na);
+    std::vector<int32_t> nlm(na);
+    for (int a = 0; a < na; ++a) {
+        pa[a] = (const double*)x.taa[a].data();
+        pb[a] = (const double*)x.tab[a].data();
+        pc[a] = (const double*)x.tbb[a].data();
+        nlm[a] = s->atoms[a].nlm;
+    }
+    return upload_M(ctx, s, nlm.data(), pa.data(), pb.data(), pc.data(), bk, bm);
+}
+
+// Append the S and H lower-tile work of one k-point.
+void add_hs_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& work,
+                     const Geometry& geo, int ng, const double2* Z, const double2* Zs,
+                     const double2* W, double2* H, double2* S, int64_t ld, int flags)
+{
+    const int nt = (int)((ng + geo.t.bm - 1) / geo.t.bm);
+    for (int which = 0; which < 2; ++which) {
+        const bool isS = which == 0;
+        if (isS && (flags & HSGEN_SKIP_S)) continue;
+        if (!isS && (flags & HSGEN_SKIP_H)) continue;
+        GemmProblem p;
+        p.L = Z;
+        p.R = isS ? Zs : W;
+        p.C = isS ? S : H;
+        p.ldl = p.ldr = 0;
+        p.ldc = ld;
+        p.m = p.n = ng;
+        p.kdepth = 0;
+        p.lower = 1;
+        const int id = (int)probs.size();
+        probs.push_back(p);
+        for (int bi = 0; bi < nt; ++bi)
+            for (int bj = 0; bj <= bi; ++bj) work.push_back(TileWork{id, bi, bj, 0});
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* hsgen_status_string(hsgen_status s)
+{
+    switch (s) {
+    case HSGEN_OK: return "ok";
+    case HSGEN_ERR_ARG: return "invalid argument";
+    case HSGEN_ERR_HIP: return "HIP runtime error";
+    case HSGEN_ERR_NOMEM: return "out of memory";
+    case HSGEN_ERR_NODEV: return "no usable gfx950 device";
+    case HSGEN_ERR_UNSUPPORTED: return "unsupported request";
+    case HSGEN_ERR_ACCURACY: return "accuracy target unreachable";
+    }
+    return "unknown status";
+}
+
+const char* hsgen_last_error(const hsgen_ctx* ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
+{
+    hsgen_ctx* ctx = nullptr;
+    if (!out) return HSGEN_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return HSGEN_ERR_NODEV;
+    }
+    if (device < 0 || device >= n) return HSGEN_ERR_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HSGEN_ERR_NODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HSGEN_ERR_NODEV;
+    ctx = new hsgen_ctx();
+    ctx->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
+    *out = ctx;
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
+{
+    if (!ctx) return HSGEN_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
+                      &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp})
+        release(*b);
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_finalize(hsgen_ctx* ctx)
+{
+    if (!ctx) return HSGEN_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    hsgen_release_workspace(ctx);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sys** out)
+{
+    if (!ctx || !d || !out) return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (d->natoms <= 0) return fail(ctx, HSGEN_ERR_ARG, "natoms must be > 0");
+    if (d->max_lmax < 1 || d->max_lmax > kMaxLmax)
+        return fail(ctx, HSGEN_ERR_ARG, "max_lmax %d outside [1, %d] (sphbes needs lmax >= 1)",
+                    d->max_lmax, kMaxLmax);
+    if (!d->lmax || !d->rmt || !d->pos || !d->rotmat || !d->match || !d->bmat || !d->t_aa ||
+        !d->t_ab || !d->t_bb)
+        return fail(ctx, HSGEN_ERR_ARG, "null array in system description");
+    if (!(d->omega > 0.0)) return fail(ctx, HSGEN_ERR_ARG, "omega must be > 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    hsgen_sys* s = new hsgen_sys();
+    s->natoms = d->natoms;
+    s->max_lmax = d->max_lmax;
+    s->nl = d->max_lmax + 1;
+    s->L = s->nl * s->nl;
+    s->omega = d->omega;
+    s->vol_prefac = 4.0 * kPi / std::sqrt(d->omega);
+    std::memcpy(s->bmat, d->bmat, sizeof s->bmat);
+    int lmx = -1;
+    int off = 0;
+    s->atoms.resize(d->natoms);
+    std::vector<RadDev> rad((size_t)d->natoms * s->nl);
+    for (int a = 0; a < d->natoms; ++a) {
+        const int lm = d->lmax[a];
+        if (lm < 0 || lm > d->max_lmax) {
+            delete s;
+            return fail(ctx, HSGEN_ERR_ARG, "lmax[%d] = %d outside [0, max_lmax]", a, lm);
+        }
+        if (!(d->rmt[a] > 0.0)) {
+            delete s;
+            return fail(ctx, HSGEN_ERR_ARG, "rmt[%d] must be > 0", a);
+        }
+        if (!d->t_aa[a] || !d->t_ab[a] || !d->t_bb[a]) {
+            delete s;
+            return fail(ctx, HSGEN_ERR_ARG, "null T block for atom %d", a);
+        }
+        lmx = std::max(lmx, lm);
+        AtomDev& at = s->atoms[a];
+        at.lmax = lm;
+        at.nlm = (lm + 1) * (lm + 1);
+        at.off = off;
+        at.pad = 0;
+        at.rmt = d->rmt[a];
+        for (int i = 0; i < 3; ++i) at.pos[i] = d->pos[3 * a + i];
+        for (int i = 0; i < 9; ++i) at.rot[i] = d->rotmat[9 * a + i];
+        off += at.nlm;
+        s->sum_nlm2 += (int64_t)at.nlm * at.nlm;
+        for (int l = 0; l < s->nl; ++l) {
+            RadDev& r = rad[(size_t)a * s->nl + l];
+            if (l <= lm) {
+                const double* m = d->match + ((size_t)a * s->nl + l) * 5;
+                r.u = m[0];
+                r.u_dr = m[1];
+                r.dE_u = m[2];
+                r.dE_u_dr = m[3];
+                r.dE_u_norm = m[4];
+                r.inv_wronk = 1.0 / (r.dE_u * r.u_dr - r.u * r.dE_u_dr);   // AtomData.cpp:63-66
+            } else {
+                r = RadDev{0, 0, 0, 0, 0, 0};
+            }
+        }
+    }
+    s->kdim = off;
+    auto yn = ynorm_table(s->max_lmax);
+    hsgen_status st = HSGEN_OK;
+    hipError_t e = hipSuccess;
+    e = hipMalloc(&s->d_atoms, sizeof(AtomDev) * s->atoms.size());
+    if (e == hipSuccess) e = hipMalloc(&s->d_rad, sizeof(RadDev) * rad.size());
+    if (e == hipSuccess) e = hipMalloc(&s->d_ynorm, sizeof(double) * yn.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(s->d_atoms, s->atoms.data(), sizeof(AtomDev) * s->atoms.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_rad, rad.data(), sizeof(RadDev) * rad.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_ynorm, yn.data(), sizeof(double) * yn.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hsgen_system_destroy(ctx, s);
+        return fail(ctx, HSGEN_ERR_HIP, "system upload: %s", hipGetErrorString(e));
+    }
+    // keep host copies of T (deduplicated later in upload_M)
+    SysExtra& x = g_extra[s];
+    x.taa.resize(d->natoms);
+    x.tab.resize(d->natoms);
+    x.tbb.resize(d->natoms);
+    std::map<const double*, int> seen;
+    for (int a = 0; a < d->natoms; ++a) {
+        const size_t n2 = (size_t)s->atoms[a].nlm * s->atoms[a].nlm;
+        auto cp = [&](std::vector<std::complex<double>>& v, const double* p) {
+            v.assign((const std::complex<double>*)p, (const std::complex<double>*)p + n2);
+        };
+        auto it = seen.find(d->t_aa[a]);
+        if (it != seen.end() && d->t_ab[a] == d->t_ab[it->second] && d->t_bb[a] == d->t_bb[it->second] &&
+            s->atoms[a].nlm == s->atoms[it->second].nlm) {
+            // share storage with the first atom of this type
+            x.taa[a] = x.taa[it->second];
+            x.tab[a] = x.tab[it->second];
+            x.tbb[a] = x.tbb[it->second];
+            continue;
+        }
+        seen[d->t_aa[a]] = a;
+        cp(x.taa[a], d->t_aa[a]);
+        cp(x.tab[a], d->t_ab[a]);
+        cp(x.tbb[a], d->t_bb[a]);
+    }
+    Geometry geo = geometry(0, s->kdim, 1);
+    st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
+    if (st != HSGEN_OK) {
+        hsgen_system_destroy(ctx, s);
+        return st;
+    }
+    *out = s;
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_system_destroy(hsgen_ctx* ctx, hsgen_sys* s)
+{
+    if (!s) return HSGEN_ERR_ARG;
+    if (ctx) (void)hipSetDevice(ctx->device);
+    if (s->d_atoms) (void)hipFree(s->d_atoms);
+    if (s->d_rad) (void)hipFree(s->d_rad);
+    if (s->d_ynorm) (void)hipFree(s->d_ynorm);
+    if (s->d_M) (void)hipFree(s->d_M);
+    g_extra.erase(s);
+    delete s;
+    return HSGEN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+hsgen_status check_kpoint(hsgen_ctx* ctx, const hsgen_kpoint* kp)
+{
+    if (!kp) return fail(ctx, HSGEN_ERR_ARG, "null k-point");
+    if (kp->ng < 0) return fail(ctx, HSGEN_ERR_ARG, "ng < 0");
+    if (kp->ng > 0 && !kp->g) return fail(ctx, HSGEN_ERR_ARG, "null g list");
+    return HSGEN_OK;
+}
+
+hsgen_status upload_g(hsgen_ctx* ctx, const hsgen_kpoint* kp)
+{
+    const size_t bytes = sizeof(int32_t) * 3 * (size_t)std::max(kp->ng, 1);
+    hsgen_status st = ensure(ctx, ctx->g, bytes, false);
+    if (st) return st;
+    if (kp->ng > 0)
+        HIPCHK(hipMemcpyAsync(ctx->g.p, kp->g, sizeof(int32_t) * 3 * (size_t)kp->ng,
+                              hipMemcpyHostToDevice, ctx->stream));
+    return HSGEN_OK;
+}
+
+AbgenParams abgen_params(const hsgen_sys* s, const hsgen_kpoint* kp, const int32_t* dg)
+{
+    AbgenParams p;
+    std::memset(&p, 0, sizeof p);
+    p.atoms = s->d_atoms;
+    p.rad = s->d_rad;
+    p.ynorm = s->d_ynorm;
+    p.g = dg;
+    for (int i = 0; i < 3; ++i) p.kpt[i] = kp->k[i];
+    for (int i = 0; i < 9; ++i) p.bmat[i] = s->bmat[i];
+    p.vol_prefac = s->vol_prefac;
+    p.natoms = s->natoms;
+    p.max_lmax = s->max_lmax;
+    p.nl = s->nl;
+    p.L = s->L;
+    p.ng = kp->ng;
+    p.kdim = s->kdim;
+    // atoms per LDS chunk: keep the block's LDS at <= 48 KiB
+    int chunk = 16;
+    while (chunk > 1 && abgen_lds_bytes(chunk, s->nl, s->L) > 48 * 1024) chunk /= 2;
+    p.chunk = chunk;
+    return p;
+}
+
+// W_a = M_a Z_a for all atoms (one grouped launch).
+void add_tmul_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& work,
+                       const hsgen_sys* s, const Geometry& geo, int ng, const double2* Z, double2* W)
+{
+    const int ntn = (int)((ng + geo.t.bn - 1) / geo.t.bn);
+    for (int a = 0; a < s->natoms; ++a) {
+        const AtomDev& at = s->atoms[a];
+        GemmProblem p;
+        p.L = s->d_M + s->m_off[a];
+        p.ldl = s->m_ld[a];
+        p.R = Z + 2 * (int64_t)at.off;
+        p.ldr = geo.ldz;
+        p.C = W + 2 * (int64_t)at.off;
+        p.ldc = geo.ldz;
+        p.m = 2 * at.nlm;
+        p.n = ng;
+        p.kdepth = s->m_ld[a];
+        p.lower = 0;
+        const int id = (int)probs.size();
+        probs.push_back(p);
+        const int ntm = (p.m + geo.t.bm - 1) / geo.t.bm;
+        for (int bi = 0; bi < ntm; ++bi)
+            for (int bj = 0; bj < ntn; ++bj) work.push_back(TileWork{id, bi, bj, 0});
+    }
+}
+
+hsgen_status run_grouped(hsgen_ctx* ctx, int cfg, std::vector<GemmProblem>& probs,
+                         std::vector<TileWork>& work)
+{
+    if (work.empty()) return HSGEN_OK;
+    hsgen_status st = ensure(ctx, ctx->probs, sizeof(GemmProblem) * probs.size(), false);
+    if (st) return st;
+    st = ensure(ctx, ctx->work, sizeof(TileWork) * work.size(), false);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(ctx->probs.p, probs.data(), sizeof(GemmProblem) * probs.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->work.p, work.data(), sizeof(TileWork) * work.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_zgemm(cfg, (const GemmProblem*)ctx->probs.p, (const TileWork*)ctx->work.p,
+                        (int)work.size(), ctx->stream));
+    // the host vectors must outlive the async copies
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return HSGEN_OK;
+}
+
+// Copy the lower triangle (and optionally everything) of a device matrix with
+// ld = n into a host matrix with leading dimension ld.
+hsgen_status copy_lower_to_host(hsgen_ctx* ctx, const double2* dsrc, int n, double* H, int64_t ld,
+                                bool full)
+{
+    std::vector<double2> tmp((size_t)n * n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), dsrc, sizeof(double2) * tmp.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double2* out = (double2*)H;
+    for (int j = 0; j < n; ++j) {
+        const int i0 = full ? 0 : j;
+        std::memcpy(out + (int64_t)j * ld + i0, tmp.data() + (size_t)j * n + i0,
+                    sizeof(double2) * (size_t)(n - i0));
+    }
+    return HSGEN_OK;
+}
+
+// Common tail of create_hs / contract_hs: Z, Zs resident -> tmul -> S, H.
+hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng, int flags,
+                       double* H, double* S, int64_t ld, hsgen_report* rep, double ms_front)
+{
+    hsgen_status st;
+    double2* Z = (double2*)ctx->Z.p;
+    double2* Zs = (double2*)ctx->Zs.p;
+    const bool need_h = !(flags & HSGEN_SKIP_H);
+    double2* W = nullptr;
+    if (need_h) {
+        st = ensure(ctx, ctx->W, sizeof(double2) * (size_t)geo.ldz * geo.gp, true);
+        if (st) return st;
+        W = (double2*)ctx->W.p;
+    }
+    const bool hdev = H ? is_device_ptr(H) : true;
+    const bool sdev = S ? is_device_ptr(S) : true;
+    double2* dH = (double2*)H;
+    double2* dS = (double2*)S;
+    int64_t ldH = ld, ldS = ld;
+    if (need_h && !hdev) {
+        st = ensure(ctx, ctx->Htmp, sizeof(double2) * (size_t)ng * ng, false);
+        if (st) return st;
+        dH = (double2*)ctx->Htmp.p;
+        ldH = ng;
+    }
+    if (!(flags & HSGEN_SKIP_S) && !sdev) {
+        st = ensure(ctx, ctx->Stmp, sizeof(double2) * (size_t)ng * ng, false);
+        if (st) return st;
+        dS = (double2*)ctx->Stmp.p;
+        ldS = ng;
+    }
+    st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
+    if (st) return st;
+
+    std::vector<GemmProblem> probs;
+    std::vector<TileWork> work;
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    if (need_h) {
+        add_tmul_problems(probs, work, s, geo, ng, Z, W);
+        st = run_grouped(ctx, geo.cfg, probs, work);
+        if (st) return st;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    probs.clear();
+    work.clear();
+    add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
+    for (auto& p : probs) {
+        p.ldl = geo.ldz;
+        p.ldr = geo.ldz;
+        p.kdepth = (int32_t)geo.kpad;
+        p.ldc = (p.C == dH) ? ldH : ldS;
+    }
+    st = run_grouped(ctx, geo.cfg, probs, work);
+    if (st) return st;
+    if (flags & HSGEN_FILL_UPPER) {
+        if (need_h) HIPCHK(launch_fill_upper(dH, ldH, ng, ctx->stream));
+        if (!(flags & HSGEN_SKIP_S)) HIPCHK(launch_fill_upper(dS, ldS, ng, ctx->stream));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev[4]));
+    if (need_h && !hdev) {
+        st = copy_lower_to_host(ctx, dH, ng, H, ld, (flags & HSGEN_FILL_UPPER) != 0);
+        if (st) return st;
+    }
+    if (!(flags & HSGEN_SKIP_S) && !sdev) {
+        st = copy_lower_to_host(ctx, dS, ng, S, ld, (flags & HSGEN_FILL_UPPER) != 0);
+        if (st) return st;
+    }
+    if (rep) {
+        float t1 = 0, t2 = 0;
+        HIPCHK(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
+        HIPCHK(hipEventElapsedTime(&t2, ctx->ev[3], ctx->ev[4]));
+        rep->ms_tmul = t1;
+        rep->ms_contract = t2;
+        rep->ms_total = ms_front + t1 + t2;
+        rep->ng = ng;
+        rep->kdim = s->kdim;
+        rep->flops_alg = 16.0 * (double)ng * ng * s->kdim + 32.0 * (double)ng * (double)s->sum_nlm2;
+        rep->slices_used = 0;
+        rep->tile_used = geo.cfg;
+    }
+    return HSGEN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hsgen_status hsgen_construct_ab(hsgen_ctx* ctx, const hsgen_sys* s, const hsgen_kpoint* kp,
+                                double* A, double* B)
+{
+    if (!ctx || !s || !A || !B) return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    hsgen_status st = check_kpoint(ctx, kp);
+    if (st) return st;
+    if (kp->ng == 0) return HSGEN_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    st = upload_g(ctx, kp);
+    if (st) return st;
+    const size_t n = (size_t)s->L * s->natoms * kp->ng;
+    const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
+    double2* dA = (double2*)A;
+    double2* dB = (double2*)B;
+    if (!adev) {
+        st = ensure(ctx, ctx->A, sizeof(double2) * n, false);
+        if (st) return st;
+        dA = (double2*)ctx->A.p;
+    }
+    if (!bdev) {
+        st = ensure(ctx, ctx->B, sizeof(double2) * n, false);
+        if (st) return st;
+        dB = (double2*)ctx->B.p;
+    }
+    AbgenParams p = abgen_params(s, kp, (const int32_t*)ctx->g.p);
+    p.ncols = kp->ng;
+    p.A = dA;
+    p.B = dB;
+    HIPCHK(launch_abgen(p, false, ctx->stream));
+    if (!adev) HIPCHK(hipMemcpyAsync(A, dA, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (!bdev) HIPCHK(hipMemcpyAsync(B, dB, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kpoint* kp,
+                             const hsgen_opts* opts, double* H, double* S, int64_t ld,
+                             hsgen_report* rep)
+{
+    if (!ctx || !sc) return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    hsgen_sys* s = const_cast<hsgen_sys*>(sc);
+    hsgen_status st = check_kpoint(ctx, kp);
+    if (st) return st;
+    hsgen_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != HSGEN_PREC_FP64)
+        return fail(ctx, HSGEN_ERR_UNSUPPORTED, "precision %d not available in this entry point", o.precision);
+    const int ng = kp->ng;
+    if (ld < std::max(ng, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld %lld < ng %d", (long long)ld, ng);
+    if ((!H && !(o.flags & HSGEN_SKIP_H)) || (!S && !(o.flags & HSGEN_SKIP_S)))
+        return fail(ctx, HSGEN_ERR_ARG, "null output matrix");
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    if (ng == 0) return HSGEN_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const Geometry geo = geometry(o.tile, s->kdim, ng);
+    st = upload_g(ctx, kp);
+    if (st) return st;
+    const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
+    st = ensure(ctx, ctx->Z, zbytes, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->Zs, zbytes, false);
+    if (st) return st;
+    AbgenParams p = abgen_params(s, kp, (const int32_t*)ctx->g.p);
+    p.ncols = (int32_t)geo.gp;
+    p.Z = (double2*)ctx->Z.p;
+    p.Zs = (double2*)ctx->Zs.p;
+    p.ldz = geo.ldz;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(launch_abgen(p, true, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    st = finish_hs(ctx, s, geo, ng, o.flags, H, S, ld, rep, 0.0);
+    if (st) return st;
+    if (rep) {
+        float t0 = 0;
+        HIPCHK(hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[1]));
+        rep->ms_abgen = t0;
+        rep->ms_total += t0;
+        rep->bytes_abgen = 2.0 * 2.0 * 16.0 * (double)ng * s->kdim;   // Z and Zs, A and B rows
+    }
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32_t nk,
+                                     const hsgen_kpoint* kps, const hsgen_opts* opts,
+                                     double* const* H, double* const* S, const int64_t* ld,
+                                     hsgen_report* rep)
+{
+    if (!ctx || !sys || nk < 0 || (nk > 0 && (!kps || !H || !S || !ld)))
+        return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    for (int i = 0; i < nk; ++i) {
+        hsgen_status st = hsgen_create_hs(ctx, sys, &kps[i], opts, H[i], S[i], ld[i], rep ? &rep[i] : nullptr);
+        if (st) return st;
+    }
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, const double* A,
+                               const double* B, const int32_t* t_size, const double* const* t_aa,
+                               const double* const* t_ab, const double* const* t_bb,
+                               const int32_t* l_maxs, int32_t max_lmax, const double* sqrt_u_norms,
+                               const hsgen_opts* opts, double* H, double* S, int64_t ld,
+                               hsgen_report* rep)
+{
+    if (!ctx) return HSGEN_ERR_ARG;
+    if (L <= 0 || a <= 0 || G < 0) return fail(ctx, HSGEN_ERR_ARG, "bad dimensions L=%d a=%d G=%d", L, a, G);
+    if (!A || !B || !t_size || !t_aa || !t_ab || !t_bb || !l_maxs || !sqrt_u_norms)
+        return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    if (max_lmax < 0 || (max_lmax + 1) * (max_lmax + 1) > L)
+        return fail(ctx, HSGEN_ERR_ARG, "max_lmax %d inconsistent with L %d", max_lmax, L);
+    hsgen_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != HSGEN_PREC_FP64)
+        return fail(ctx, HSGEN_ERR_UNSUPPORTED, "precision %d not available in this entry point", o.precision);
+    if (ld < std::max(G, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld < G");
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    HIPCHK(hipSetDevice(ctx->device));
+    // a throw-away system holding only what the contraction needs
+    hsgen_sys s;
+    s.natoms = a;
+    s.max_lmax = max_lmax;
+    s.nl = max_lmax + 1;
+    s.L = L;
+    int off = 0;
+    s.atoms.resize(a);
+    std::vector<double> un2((size_t)a * s.nl, 0.0);
+    for (int i = 0; i < a; ++i) {
+        const int lm = l_maxs[i];
+        if (lm < 0 || lm > max_lmax) return fail(ctx, HSGEN_ERR_ARG, "l_maxs[%d] out of range", i);
+        const int nlm = (lm + 1) * (lm + 1);
+        if (t_size[i] != nlm)
+            return fail(ctx, HSGEN_ERR_ARG, "t_size[%d] = %d but (l_maxs+1)^2 = %d", i, t_size[i], nlm);
+        AtomDev& at = s.atoms[i];
+        std::memset(&at, 0, sizeof at);
+        at.lmax = lm;
+        at.nlm = nlm;
+        at.off = off;
+        off += nlm;
+        s.sum_nlm2 += (int64_t)nlm * nlm;
+        for (int l = 0; l <= lm; ++l) {
+            const double v = sqrt_u_norms[(size_t)i * s.nl + l];
+            un2[(size_t)i * s.nl + l] = v * v;
+        }
+    }
+    s.kdim = off;
+    if (G == 0) return HSGEN_OK;
+    const Geometry geo = geometry(o.tile, s.kdim, G);
+    hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);
+    if (st) return st;
+    struct Guard {
+        hsgen_sys* s;
+        ~Guard() { if (s->d_M) (void)hipFree(s->d_M); s->d_M = nullptr; }
+    } guard{&s};
+    // device copies of A, B if they are host memory
+    const size_t nab = (size_t)L * a * G;
+    const double2* dA = (const double2*)A;
+    const double2* dB = (const double2*)B;
+    if (!is_device_ptr(A)) {
+        st = ensure(ctx, ctx->A, sizeof(double2) * nab, false);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(ctx->A.p, A, sizeof(double2) * nab, hipMemcpyHostToDevice, ctx->stream));
+        dA = (const double2*)ctx->A.p;
+    }
+    if (!is_device_ptr(B)) {
+        st = ensure(ctx, ctx->B, sizeof(double2) * nab, false);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(ctx->B.p, B, sizeof(double2) * nab, hipMemcpyHostToDevice, ctx->stream));
+        dB = (const double2*)ctx->B.p;
+    }
+    st = ensure(ctx, ctx->atoms_tmp, sizeof(AtomDev) * a, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->unorm2, sizeof(double) * un2.size(), false);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(ctx->atoms_tmp.p, s.atoms.data(), sizeof(AtomDev) * a, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->unorm2.p, un2.data(), sizeof(double) * un2.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
+    st = ensure(ctx, ctx->Z, zbytes, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->Zs, zbytes, false);
+    if (st) return st;
+    PackParams pp;
+    pp.A = dA;
+    pp.B = dB;
+    pp.atoms = (const AtomDev*)ctx->atoms_tmp.p;
+    pp.unorm2 = (const double*)ctx->unorm2.p;
+    pp.natoms = a;
+    pp.L = L;
+    pp.nl = s.nl;
+    pp.ng = G;
+    pp.kdim = s.kdim;
+    pp.ldab = (int64_t)L * a;
+    pp.Z = (double2*)ctx->Z.p;
+    pp.Zs = (double2*)ctx->Zs.p;
+    pp.ldz = geo.ldz;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(launch_pack_ref(pp, (int)geo.gp, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    st = finish_hs(ctx, &s, geo, G, o.flags, H, S, ld, rep, 0.0);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (st) return st;
+    if (rep) {
+        float t0 = 0;
+        HIPCHK(hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[1]));
+        rep->ms_abgen = t0;
+        rep->ms_total += t0;
+    }
+    return HSGEN_OK;
+}
+
+}  // extern "C"

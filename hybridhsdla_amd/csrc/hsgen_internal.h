@@ -1,0 +1,99 @@
+// Internal declarations shared by the hsgen HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hsgen {
+
+constexpr int kMaxLmax = 24;            // largest supported l cutoff
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Per-atom data of the A/B generator (AtomData fields, AtomData.h:16-52).
+struct AtomDev {
+    int32_t lmax;      // spherical cutoff of this atom
+    int32_t nlm;       // (lmax+1)^2
+    int32_t off;       // sum of nlm over preceding atoms (row offset / 2 in Z)
+    int32_t pad;
+    double rmt;        // muffin-tin radius
+    double pos[3];     // fractional position
+    double rot[9];     // column-major R_atom * bmat
+};
+
+// Per-(atom, l) radial matching data (rad_match, types.h:35-43) plus the
+// inverse Wronskian (AtomData.cpp:63-66).
+struct RadDev {
+    double u, u_dr, dE_u, dE_u_dr;
+    double inv_wronk;
+    double dE_u_norm;
+};
+
+// One problem of the grouped complex GEMM  C = L^H R  (complex128).
+//   L: kdepth x (>= m) columns, leading dimension ldl (complex units)
+//   R: kdepth x (>= n) columns, leading dimension ldr
+//   C: m x n, leading dimension ldc; element (i, j) at C[i + j*ldc]
+// kdepth is a multiple of the kernel's BK and the operands are readable
+// (zero padded) up to the tile-rounded column counts.  lower != 0: only
+// tiles bi >= bj exist and only i >= j is stored, with Im(C_ii) := 0.
+struct GemmProblem {
+    const double2* L;
+    const double2* R;
+    double2* C;
+    int64_t ldl, ldr, ldc;
+    int32_t m, n, kdepth, lower;
+};
+
+struct TileWork {
+    int32_t prob, bi, bj, pad;
+};
+
+// Parameters of the A/B generator (abgen.hip).
+struct AbgenParams {
+    const AtomDev* atoms;
+    const RadDev* rad;        // [natoms][nl]
+    const double* ynorm;      // [L] ylm4 normalisation (incl. (-1)^m for m < 0)
+    const int32_t* g;         // [ng][3]
+    double kpt[3];
+    double bmat[9];           // column-major
+    double vol_prefac;        // 4 pi / sqrt(Omega)
+    int32_t natoms, max_lmax, nl, L;
+    int32_t ng, ncols;        // columns written (compact: padded; ref: ng)
+    int32_t chunk;            // atoms per LDS chunk
+    int32_t kdim;             // K = sum nlm (compact)
+    // compact
+    double2* Z;
+    double2* Zs;
+    int64_t ldz;
+    // ref
+    double2* A;
+    double2* B;
+};
+
+// Pack reference-layout A, B (L*a x G, ld = L*a) into the compact operands
+// Z = [A_a; B_a] per atom and Zs = [A_a; N_a B_a], N = sqrt_u_norms^2
+// (h_and_s_stripped_v2.cpp:64-86 applies sqrt_u_norms to B before the herk).
+struct PackParams {
+    const double2* A;
+    const double2* B;
+    const AtomDev* atoms;
+    const double* unorm2;   // [natoms][nl] squared sqrt_u_norms
+    int32_t natoms, L, nl, ng, kdim;
+    int64_t ldab;           // L * natoms
+    double2* Z;
+    double2* Zs;
+    int64_t ldz;
+};
+
+hipError_t launch_abgen(const AbgenParams& p, bool compact, hipStream_t st);
+size_t abgen_lds_bytes(int chunk, int nl, int L);
+hipError_t launch_pack_ref(const PackParams& p, int ncols, hipStream_t st);
+
+struct ZgemmLaunchCfg {
+    int bm, bn, bk, threads;
+};
+ZgemmLaunchCfg zgemm_cfg(int id);
+hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
+                        hipStream_t st);
+hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st);
+
+}  // namespace hsgen

@@ -1,0 +1,217 @@
+// Grouped complex128 GEMM  C = L^H R  on CDNA4 FP64 MFMA (v_mfma_f64_16x16x4).
+//
+// One launch serves a list of tiles drawn from several problems (the S and H
+// contractions of one or many k-points, or the per-atom T application), so
+// uneven problem sizes share one grid.  Lower-triangular problems contribute
+// only tiles bi >= bj and store only i >= j, which is what the reference's
+// zherk/zher2k/zgemmt calls with uplo='L' produce.
+//
+// Complex arithmetic on the real MFMA: with l = conj(L[k,i]) and r = R[k,j],
+//   Re C += Lr.Rr + Li.Ri      Im C += Lr.Ri - Li.Rr
+// i.e. four 16x16x4 real MFMAs per 16x16 complex block and k-step of 4, each
+// lane feeding (Lr, Li) and (Rr, Ri) read as one 16-byte LDS word.
+//
+// MFMA operand layout (verified on gfx950): A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k, D[i][j] in lane j + 16(i % 4), register i / 4.
+//
+// Tiles: BM x BN complex per workgroup, WGM x WGN waves, BK complex of depth
+// per LDS stage, double-buffered; global loads of stage s+1 are issued before
+// the MFMAs of stage s.  LDS holds each operand column's BK complex values
+// contiguously plus one complex of padding, which makes the 16-lane column
+// reads bank-conflict free.
+#include "hsgen_internal.h"
+
+namespace hsgen {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int BK, int WGM, int WGN>
+struct ZgemmCfg {
+    static constexpr int kThreads = WGM * WGN * 64;
+    static constexpr int kWM = BM / WGM;
+    static constexpr int kWN = BN / WGN;
+    static constexpr int kRB = kWM / 16;
+    static constexpr int kCB = kWN / 16;
+    static constexpr int kColStride = 2 * BK + 2;                 // doubles per column slot
+    static constexpr int kStageDoubles = (BM + BN) * kColStride;
+    static constexpr int kLoads = (BM + BN) * BK / kThreads;      // complex loads per thread per stage
+    static_assert(kWM % 16 == 0 && kWN % 16 == 0, "wave tile must be a multiple of 16");
+    static_assert(BK % 4 == 0, "BK must be a multiple of 4");
+    static_assert((BM + BN) * BK % kThreads == 0, "stage must split evenly over threads");
+};
+
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* __restrict__ probs,
+                                                             const TileWork* __restrict__ work,
+                                                             int nwork)
+{
+    using Cfg = ZgemmCfg<BM, BN, BK, WGM, WGN>;
+    constexpr int NT = Cfg::kThreads;
+    constexpr int RB = Cfg::kRB, CB = Cfg::kCB;
+    constexpr int CS = Cfg::kColStride;
+    constexpr int NL = Cfg::kLoads;
+    __shared__ double lds[2 * Cfg::kStageDoubles];
+
+    const int wid = blockIdx.x;
+    if (wid >= nwork) return;
+    const TileWork tw = work[wid];
+    const GemmProblem pb = probs[tw.prob];
+    const int I0 = tw.bi * BM;
+    const int J0 = tw.bj * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave % WGM;
+    const int wn = wave / WGM;
+
+    // per-thread global source pointers (complex units) for each of its loads
+    const double2* src[NL];
+    int ldsoff[NL];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+        const int e = tid + NT * r;
+        const int c = e / BK;
+        const int kk = e - c * BK;
+        if (c < BM) src[r] = pb.L + (int64_t)(I0 + c) * pb.ldl + kk;
+        else        src[r] = pb.R + (int64_t)(J0 + c - BM) * pb.ldr + kk;
+        ldsoff[r] = c * CS + 2 * kk;
+    }
+
+    d4 cr[RB][CB], ci[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            cr[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+            ci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+
+    const int nst = pb.kdepth / BK;
+    double2 stage[NL];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) stage[r] = src[r][0];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) *(double2*)&lds[ldsoff[r]] = stage[r];
+    __syncthreads();
+
+    // lane's LDS read offsets inside a stage
+    const int rowA = (wm * Cfg::kWM + (lane & 15)) * CS + 2 * (lane >> 4);
+    const int rowB = (BM + wn * Cfg::kWN + (lane & 15)) * CS + 2 * (lane >> 4);
+
+    for (int s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        const bool more = (s + 1) < nst;
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < NL; ++r) stage[r] = src[r][(int64_t)(s + 1) * BK];
+        }
+        const double* lb = lds + buf * Cfg::kStageDoubles;
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            double2 a[RB], b[CB];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) a[i] = *(const double2*)&lb[rowA + i * 16 * CS + 8 * ks];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) b[j] = *(const double2*)&lb[rowB + j * 16 * CS + 8 * ks];
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, cr[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, ci[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, cr[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const double nai = -a[i].y;
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].x, ci[i][j], 0, 0, 0);
+            }
+        }
+        if (more) {
+            double* nb = lds + (buf ^ 1) * Cfg::kStageDoubles;
+#pragma unroll
+            for (int r = 0; r < NL; ++r) *(double2*)&nb[ldsoff[r]] = stage[r];
+        }
+        __syncthreads();
+    }
+
+    // epilogue: lane holds D[(lane>>4) + 4 q][lane & 15] in register q
+    const bool lower = pb.lower != 0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int gj = J0 + wn * Cfg::kWN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = I0 + wm * Cfg::kWM + i * 16 + (lane >> 4) + 4 * q;
+                if (gi < pb.m && gj < pb.n && (!lower || gi >= gj)) {
+                    double im = ci[i][j][q];
+                    if (lower && gi == gj) im = 0.0;
+                    pb.C[(int64_t)gj * pb.ldc + gi] = make_double2(cr[i][j][q], im);
+                }
+            }
+        }
+}
+
+// Tile configurations (id -> template instance).
+
+#define HSGEN_ZGEMM_CFGS(X)        \
+    X(1, 128, 128, 8, 2, 4)         \
+    X(2, 128, 128, 16, 2, 4)        \
+    X(3, 64, 64, 8, 2, 2)           \
+    X(4, 128, 64, 8, 2, 2)
+
+ZgemmLaunchCfg zgemm_cfg(int id)
+{
+#define X(ID, BM, BN, BK, WGM, WGN) \
+    if (id == ID) return ZgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64};
+    HSGEN_ZGEMM_CFGS(X)
+#undef X
+    return ZgemmLaunchCfg{0, 0, 0, 0};
+}
+
+hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
+                        hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+#define X(ID, BM, BN, BK, WGM, WGN)                                                      \
+    if (id == ID) {                                                                      \
+        hipLaunchKernelGGL((k_zgemm_lh<BM, BN, BK, WGM, WGN>), dim3(nwork),             \
+                           dim3(WGM * WGN * 64), 0, st, probs, work, nwork);             \
+        return hipGetLastError();                                                        \
+    }
+    HSGEN_ZGEMM_CFGS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// Mirror the lower triangle into the upper one: C[j + i*ld] = conj(C[i + j*ld]).
+__global__ void k_fill_upper(double2* C, int64_t ld, int n)
+{
+    const int j = blockIdx.y * 16 + threadIdx.y;
+    const int i = blockIdx.x * 16 + threadIdx.x;
+    if (i < n && j < n && i < j) {
+        double2 v = C[(int64_t)i * ld + j];
+        C[(int64_t)j * ld + i] = make_double2(v.x, -v.y);
+    }
+}
+
+hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    dim3 grid((n + 15) / 16, (n + 15) / 16), block(16, 16);
+    hipLaunchKernelGGL(k_fill_upper, grid, block, 0, st, C, ld, n);
+    return hipGetLastError();
+}
+
+}  // namespace hsgen
