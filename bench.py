@@ -1,0 +1,274 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per H+S and effective TFLOP/s on one MI355X.
+
+A step is one full H+S generation for one k-point of the workload (A/B
+generation, per-atom T application, lower-triangle S and H contraction), with
+the system description already resident in HBM and H, S written to HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+                    [--precision fp64|int8] [--no-cpu] [--no-check]
+
+Prints ONE JSON line.  The default workload is the 256-atom, lmax 12,
+N_G ~ 25000 system (FP64).  Effective TFLOP/s uses the algorithmic work
+F = 16 G^2 K + 32 G sum_a L_a^2 (complex MAC = 8 flops, lower triangle).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "sec per H+S & TFLOP/s: FP64 MFMA vs split-precision at ≤1e-12 rel err"
+FP64_PEAK = 78.6      # TFLOP/s, MI355X dense FP64 matrix
+INT8_PEAK = 5000.0    # TOP/s, MI355X dense INT8
+HBM_PEAK = 8000.0     # GB/s
+
+WORKLOAD = {
+    0: "cfg0: 4 atoms/2 types, lmax 8 (L=81), a=10 bohr, k=(0.1,0.2,0.3)",
+    1: "cfg1: 32 atoms/4 types, lmax 10 (L=121), a=20 bohr, k=(0.1,0.2,0.3)",
+    2: "cfg2: 256 atoms/8 types, lmax 12 (L=169), a=40 bohr, k=(0.1,0.2,0.3)",
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(cfg: int, budget_s: float):
+    """Time the reference CPU path (oracle: serial construct_AB + the v2 BLAS
+    sequence on the host OpenBLAS) on the same system at a reduced plane-wave
+    count sized to ~budget_s of CPU work."""
+    import numpy as np
+    from hybridhsdla_amd.system import gvectors, kmax_for_ng, make_config, KPoint
+    from oracle import pyoracle as po
+
+    threads = cpu_threads()
+    if not po.blas_ready(threads):
+        return None
+    system, kps = make_config(cfg)
+    full_ng = kps[0].ng
+    k = kps[0].k
+    b33 = system.bmat33()
+
+    def run(ng):
+        km = kmax_for_ng(k, b33, ng)
+        kp = KPoint(k=k, g=gvectors(k, b33, km), kmax=km)
+        t0 = time.perf_counter()
+        po.reference_hs(system, kp, threads=threads)
+        return time.perf_counter() - t0, kp.ng
+
+    # calibrate on a small sample, then size the real sample to the budget
+    t_small, ng_small = run(min(400, full_ng))
+    f_small = system.flops_alg(ng_small)
+    rate = f_small / max(t_small, 1e-6)
+    ng_s = ng_small
+    for cand in (800, 1200, 1600, 2000, 2500, 3000, 4000, 5000, 6000, 8000):
+        if cand > full_ng:
+            break
+        if system.flops_alg(cand) / rate <= budget_s:
+            ng_s = cand
+    t_s, ng_real = run(ng_s) if ng_s != ng_small else (t_small, ng_small)
+    f_s = system.flops_alg(ng_real)
+    tflops = f_s / t_s / 1e12
+    return {
+        "value": round(tflops, 5),
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{WORKLOAD.get(cfg, cfg)} system at N_G={ng_real} of {full_ng} "
+                   f"(smaller |k+G| cutoff); serial construct_AB + v2 BLAS sequence "
+                   f"(OpenBLAS, {threads} threads); {t_s:.2f} s"),
+        "sec_sample": round(t_s, 4),
+        "sec_per_hs_extrapolated": round(system.flops_alg(full_ng) / (tflops * 1e12), 2),
+    }
+
+
+def sampled_parity(system, kp, H_dev, S_dev, ld, nsample=48, seed=1):
+    """Relative Frobenius error of H and S on a random principal submatrix
+    (nsample G-vectors) against the oracle evaluated on exactly those
+    G-vectors."""
+    import numpy as np
+    import torch
+    from hybridhsdla_amd.system import KPoint
+    from oracle import pyoracle as po
+
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(kp.ng, size=min(nsample, kp.ng), replace=False))
+    sub = KPoint(k=kp.k, g=kp.g[idx], kmax=kp.kmax)
+    Hr, Sr = po.reference_hs(system, sub, threads=cpu_threads())
+    ti = torch.as_tensor(idx, device=H_dev.device)
+    # device buffers hold column-major matrices with leading dimension ld:
+    # torch row r = column r of the matrix
+    Hs = H_dev[ti][:, ti].cpu().numpy().T
+    Ss = S_dev[ti][:, ti].cpu().numpy().T
+
+    def rel(x, r):
+        d = np.tril(x - r)
+        return float(np.linalg.norm(d) / np.linalg.norm(np.tril(r)))
+
+    return {"rel_err_H": rel(Hs, Hr), "rel_err_S": rel(Ss, Sr), "n_sample": int(idx.size)}
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM traffic per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / f"{kernel}_{workload}_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())
+    except Exception:
+        return None
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "int8"])
+    ap.add_argument("--slices", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the H/S path is benchmarked on a "
+              "single MI355X); --gpus must be 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+
+    t0 = time.perf_counter()
+    system, kps = make_config(args.config)
+    kp = kps[0]
+    G = kp.ng
+    log(f"[bench] workload cfg{args.config}: natoms={system.natoms} K={system.kdim} G={G} "
+        f"kmax={kp.kmax:.4f} (setup {time.perf_counter() - t0:.1f} s)")
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    sysh = ctx.upload(system)
+    H = torch.zeros((G, G), dtype=torch.complex128, device=dev)
+    S = torch.zeros((G, G), dtype=torch.complex128, device=dev)
+    prec = HSGEN_PREC_FP64 if args.precision == "fp64" else HSGEN_PREC_OZAKI_INT8
+
+    def step():
+        return ctx.create_hs_device(sysh, kp, H.data_ptr(), S.data_ptr(), G, precision=prec,
+                                    slices=args.slices, tile=args.tile)
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"[bench] warmup {i}: {r['ms_total']:.1f} ms (abgen {r['ms_abgen']:.2f}, tmul "
+            f"{r['ms_tmul']:.2f}, contract {r['ms_contract']:.1f})")
+    torch.cuda.synchronize()
+    reps = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        reps.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed * 1e3 / args.steps
+    log(f"[bench] {args.steps} steps: {ms_step:.1f} ms/step")
+
+    F = system.flops_alg(G)
+    tflops = F / (ms_step * 1e-3) / 1e12
+    ms_contract = float(np.mean([r["ms_contract"] for r in reps]))
+    ms_abgen = float(np.mean([r["ms_abgen"] for r in reps]))
+    ms_tmul = float(np.mean([r["ms_tmul"] for r in reps]))
+    f_contract = 16.0 * G * G * system.kdim
+    contract_tf = f_contract / (ms_contract * 1e-3) / 1e12
+    peak = FP64_PEAK if prec == HSGEN_PREC_FP64 else INT8_PEAK
+    workload_tag = f"cfg{args.config}"
+    traffic = load_traffic("zgemm_lh_contract", workload_tag)
+    roofline = {
+        "bound": "mfma",
+        "kernel": "k_zgemm_lh (S and H lower-triangle tiles, one grouped launch)",
+        "achieved": round(contract_tf, 3),
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": round(contract_tf / peak, 4),
+        "traffic": (traffic or {}).get("bytes_per_launch"),
+        "algorithmic_flops_per_launch": f_contract,
+        "avg_launch_ms": round(ms_contract, 3),
+    }
+    abgen_bytes = float(reps[-1]["bytes_abgen"])
+    abgen = {
+        "bound": "hbm", "kernel": "k_abgen (A/B generator)",
+        "achieved": round(abgen_bytes / (ms_abgen * 1e-3) / 1e9, 1) if ms_abgen > 0 else None,
+        "peak": HBM_PEAK, "unit": "GB/s",
+        "frac": round(abgen_bytes / (ms_abgen * 1e-3) / 1e9 / HBM_PEAK, 4) if ms_abgen > 0 else None,
+        "bytes_per_launch": abgen_bytes, "avg_launch_ms": round(ms_abgen, 4),
+    }
+
+    check = None
+    if not args.no_check:
+        try:
+            check = sampled_parity(system, kp, H, S, G)
+            log(f"[bench] sampled parity: {check}")
+        except Exception as e:  # the check must not hide the measurement
+            check = {"error": str(e)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.config, args.cpu_budget)
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(tflops, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "sec_per_hs": round(ms_step * 1e-3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == HSGEN_PREC_FP64 else "int8-slices/f64",
+        "data": "synthetic (seeded system generator; random-init T-matrices and radial values)",
+        "config": {
+            "workload": WORKLOAD[args.config] + f", N_G={G} (Kmax={kp.kmax:.4f}), K={system.kdim}",
+            "natoms": system.natoms, "n_g": G, "k_dim": system.kdim,
+            "precision": args.precision, "tile": reps[-1]["tile_used"],
+        },
+        "flops_per_step": F,
+        "breakdown_ms": {"abgen": round(ms_abgen, 3), "tmul": round(ms_tmul, 3),
+                         "contract": round(ms_contract, 3)},
+        "roofline": roofline,
+        "abgen_roofline": abgen,
+        "parity_sample": check,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    sysh.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -25,7 +26,7 @@
 
 using namespace hsgen;
 
-static const int kDefaultTile = 1;
+static const int kDefaultTile = 8;
 
 struct DevBuf {
     void* p = nullptr;
@@ -253,7 +254,9 @@ void add_hs_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& wor
                      const Geometry& geo, int ng, const double2* Z, const double2* Zs,
                      const double2* W, double2* H, double2* S, int64_t ld, int flags)
 {
-    const int nt = (int)((ng + geo.t.bm - 1) / geo.t.bm);
+    const int ntr = (int)((ng + geo.t.bm - 1) / geo.t.bm);   // tile rows
+    const int ntc = (int)((ng + geo.t.bn - 1) / geo.t.bn);   // tile columns
+    int ids[2] = {-1, -1};
     for (int which = 0; which < 2; ++which) {
         const bool isS = which == 0;
         if (isS && (flags & HSGEN_SKIP_S)) continue;
@@ -267,13 +270,42 @@ void add_hs_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& wor
         p.m = p.n = ng;
         p.kdepth = 0;
         p.lower = 1;
-        const int id = (int)probs.size();
+        ids[which] = (int)probs.size();
         probs.push_back(p);
-        for (int bi = 0; bi < nt; ++bi)
-            for (int bj = 0; bj <= bi; ++bj) work.push_back(TileWork{id, bi, bj, 0});
     }
+    // Logical order: patches of kPatchR x kPatchC tiles, S and H tiles of a
+    // patch next to each other (they share the Z row panel).  Tiles exist
+    // where the tile's first row index >= its first column index.
+    const int kPatchR = 4, kPatchC = 4;
+    for (int pi = 0; pi < ntr; pi += kPatchR)
+        for (int pj = 0; pj < ntc; pj += kPatchC)
+            for (int which = 0; which < 2; ++which) {
+                if (ids[which] < 0) continue;
+                for (int bi = pi; bi < std::min(ntr, pi + kPatchR); ++bi)
+                    for (int bj = pj; bj < std::min(ntc, pj + kPatchC); ++bj)
+                        if ((int64_t)bj * geo.t.bn < (int64_t)(bi + 1) * geo.t.bm)
+                            work.push_back(TileWork{ids[which], bi, bj, 0});
+            }
 }
 
+// Permute a logical tile list so that each XCD runs consecutive logical tiles.
+// Workgroups are dispatched round-robin over the 8 XCDs in blockIdx order, so
+// in a window of 8*32 blocks XCD x runs blocks x, x+8, ...; giving those the
+// logical tiles [32*(8w+x), 32*(8w+x)+32) keeps one patch per XCD L2.
+void xcd_permute(std::vector<TileWork>& work)
+{
+    const char* env = std::getenv("HSGEN_XCD_REMAP");
+    if (env && env[0] == '0') return;
+    const size_t kXcd = 8, kPer = 32, kWin = kXcd * kPer;
+    const size_t n = work.size();
+    const size_t full = n / kWin * kWin;
+    std::vector<TileWork> out(work);
+    for (size_t b = 0; b < full; ++b) {
+        const size_t w = b / kWin, x = b % kXcd, slot = (b % kWin) / kXcd;
+        out[b] = work[w * kWin + x * kPer + slot];
+    }
+    work.swap(out);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -633,6 +665,7 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
     probs.clear();
     work.clear();
     add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
+    xcd_permute(work);
     for (auto& p : probs) {
         p.ldl = geo.ldz;
         p.ldr = geo.ldz;

@@ -24,6 +24,8 @@
 namespace hsgen {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));   // one complex128 (re, im)
+typedef __attribute__((address_space(1))) const v2d* gptr;
 
 template <int BM, int BN, int BK, int WGM, int WGN>
 struct ZgemmCfg {
@@ -64,16 +66,20 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
     const int wm = wave % WGM;
     const int wn = wave / WGM;
 
-    // per-thread global source pointers (complex units) for each of its loads
-    const double2* src[NL];
+    // per-thread global source pointers (complex units) for each of its
+    // loads, in the global address space so that the loads are global_load
+    // (counted by vmcnt only) rather than flat loads, which would also hold
+    // up every wait on the LDS reads
+    gptr src[NL];
     int ldsoff[NL];
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
         const int e = tid + NT * r;
         const int c = e / BK;
         const int kk = e - c * BK;
-        if (c < BM) src[r] = pb.L + (int64_t)(I0 + c) * pb.ldl + kk;
-        else        src[r] = pb.R + (int64_t)(J0 + c - BM) * pb.ldr + kk;
+        const double2* p = (c < BM) ? pb.L + (int64_t)(I0 + c) * pb.ldl + kk
+                                    : pb.R + (int64_t)(J0 + c - BM) * pb.ldr + kk;
+        src[r] = (gptr)(const void*)p;
         ldsoff[r] = c * CS + 2 * kk;
     }
 
@@ -87,11 +93,11 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
         }
 
     const int nst = pb.kdepth / BK;
-    double2 stage[NL];
+    v2d stage[NL];
 #pragma unroll
     for (int r = 0; r < NL; ++r) stage[r] = src[r][0];
 #pragma unroll
-    for (int r = 0; r < NL; ++r) *(double2*)&lds[ldsoff[r]] = stage[r];
+    for (int r = 0; r < NL; ++r) *(v2d*)&lds[ldsoff[r]] = stage[r];
     __syncthreads();
 
     // lane's LDS read offsets inside a stage
@@ -100,19 +106,23 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
 
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
-        const bool more = (s + 1) < nst;
-        if (more) {
+        // Loads of the next stage go out before this stage's MFMAs and are
+        // only waited for when they are written to LDS afterwards.  The last
+        // stage re-reads itself (in bounds, result unused) to keep the loop
+        // free of branches.
+        const int64_t knext = (int64_t)min(s + 1, nst - 1) * BK;
 #pragma unroll
-            for (int r = 0; r < NL; ++r) stage[r] = src[r][(int64_t)(s + 1) * BK];
-        }
+        for (int r = 0; r < NL; ++r) stage[r] = src[r][knext];
+        // keep the scheduler from sinking the loads down to their LDS store
+        __builtin_amdgcn_sched_barrier(0);
         const double* lb = lds + buf * Cfg::kStageDoubles;
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
-            double2 a[RB], b[CB];
+            v2d a[RB], b[CB];
 #pragma unroll
-            for (int i = 0; i < RB; ++i) a[i] = *(const double2*)&lb[rowA + i * 16 * CS + 8 * ks];
+            for (int i = 0; i < RB; ++i) a[i] = *(const v2d*)&lb[rowA + i * 16 * CS + 8 * ks];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) b[j] = *(const double2*)&lb[rowB + j * 16 * CS + 8 * ks];
+            for (int j = 0; j < CB; ++j) b[j] = *(const v2d*)&lb[rowB + j * 16 * CS + 8 * ks];
 #pragma unroll
             for (int i = 0; i < RB; ++i)
 #pragma unroll
@@ -136,11 +146,11 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
                     ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].x, ci[i][j], 0, 0, 0);
             }
         }
-        if (more) {
-            double* nb = lds + (buf ^ 1) * Cfg::kStageDoubles;
+        // ... and from hoisting the wait for them above the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+        double* nb = lds + (buf ^ 1) * Cfg::kStageDoubles;
 #pragma unroll
-            for (int r = 0; r < NL; ++r) *(double2*)&nb[ldsoff[r]] = stage[r];
-        }
+        for (int r = 0; r < NL; ++r) *(v2d*)&nb[ldsoff[r]] = stage[r];
         __syncthreads();
     }
 
@@ -169,7 +179,11 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
     X(1, 128, 128, 8, 2, 4)         \
     X(2, 128, 128, 16, 2, 4)        \
     X(3, 64, 64, 8, 2, 2)           \
-    X(4, 128, 64, 8, 2, 2)
+    X(4, 128, 64, 8, 2, 2)          \
+    X(5, 128, 128, 8, 2, 2)         \
+    X(6, 128, 128, 8, 4, 2)         \
+    X(7, 128, 128, 16, 2, 2)        \
+    X(8, 128, 128, 16, 4, 2)
 
 ZgemmLaunchCfg zgemm_cfg(int id)
 {

@@ -97,7 +97,7 @@ def test_tile_configs_agree(ctx):
     s, kp = tiny_system(natoms=3, lmax=(3, 2, 4), ng_target=200)
     h = ctx.upload(s)
     H1, S1 = ctx.create_HS2(h, kp, tile=1)
-    for t in (2, 3, 4):
+    for t in (2, 3, 4, 5, 6, 7, 8):
         H2, S2 = ctx.create_HS2(h, kp, tile=t)
         assert rel_fro_lower(H2, H1) <= 1e-14
         assert rel_fro_lower(S2, S1) <= 1e-14
