@@ -40,6 +40,8 @@ struct hsgen_ctx {
     std::string err;
     // workspace
     DevBuf Z, Zs, W, Htmp, Stmp, A, B, g, probs, work, unorm2, atoms_tmp;
+    // split precision
+    DevBuf oz_e, oz_res, oz_dd, oz_probs, oz_work;
 };
 
 struct hsgen_sys {
@@ -363,7 +365,8 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
-                      &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp})
+                      &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp, &ctx->oz_e, &ctx->oz_res,
+                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work})
         release(*b);
     return HSGEN_OK;
 }
@@ -619,10 +622,236 @@ hsgen_status copy_lower_to_host(hsgen_ctx* ctx, const double2* dsrc, int n, doub
     return HSGEN_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Split precision (Ozaki scheme II on INT8 MFMA)
+// ---------------------------------------------------------------------------
+// Pairwise-coprime moduli <= 255 (symmetric residues fit INT8), largest first.
+const int kModuli[kMaxModuli] = {255, 254, 253, 251, 247, 241, 239, 233, 229, 227,
+                                 223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+const int kIgemmCfg = 1;
+
+struct OzPlan {
+    int nmod = 0;
+    int beta = 0;
+    double log2P = 0.0;
+    double Phi = 0.0, Plo = 0.0;
+    double whi[kMaxModuli], wlo[kMaxModuli];
+};
+
+int inv_mod(int a, int m)
+{
+    a %= m;
+    if (a < 0) a += m;
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+
+// beta for n moduli and depth kd: P > 2 * (2 kd) * 2^(2 beta)
+int oz_beta(int n, int64_t kd)
+{
+    double l2p = 0.0;
+    for (int m = 0; m < n; ++m) l2p += std::log2((double)kModuli[m]);
+    return (int)std::floor((l2p - 2.0 - std::log2(2.0 * (double)kd)) / 2.0);
+}
+
+// Number of moduli for a relative error target.  Rounding the scaled
+// operands to beta bits perturbs each element of C by ~2^-beta |L_i| |R_j|,
+// so the relative Frobenius error is ~c 2^-beta with c growing like the
+// square root of the matrix dimension (H cancels more than S, its T being
+// indefinite); three bits of margin on top.
+int oz_moduli_for(double target, int64_t kd, int ng)
+{
+    if (!(target > 0.0)) target = 1e-12;
+    const double dim = (double)std::max<int64_t>(kd, ng);
+    const int need = (int)std::ceil(-std::log2(target) + 0.5 * std::log2(dim) + 3.0);
+    for (int n = 2; n <= kMaxModuli; ++n)
+        if (oz_beta(n, kd) >= need) return n;
+    return -1;
+}
+
+OzPlan oz_plan(int nmod, int64_t kd)
+{
+    OzPlan pl;
+    pl.nmod = nmod;
+    pl.beta = oz_beta(nmod, kd);
+    // P as double-double (exact while the product fits 106 bits)
+    double ph = 1.0, plo = 0.0;
+    for (int m = 0; m < nmod; ++m) {
+        const double p = kModuli[m];
+        const double h = ph * p;
+        const double e = std::fma(ph, p, -h);
+        const double l = plo * p + e;
+        ph = h + l;
+        plo = l - (ph - h);
+        pl.log2P += std::log2(p);
+    }
+    pl.Phi = ph;
+    pl.Plo = plo;
+    for (int m = 0; m < nmod; ++m) {
+        int u = 1;
+        for (int k = 0; k < nmod; ++k)
+            if (k != m) u = (u * inv_mod(kModuli[k], kModuli[m])) % kModuli[m];
+        const double p = kModuli[m];
+        const double hi = (double)u / p;
+        const double r = std::fma(-hi, p, (double)u);   // exact remainder
+        pl.whi[m] = hi;
+        pl.wlo[m] = r / p;
+    }
+    return pl;
+}
+
+// S = Z^H Zs and H = Z^H W from the resident FP64 operands, on INT8 MFMA.
+hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod_req,
+                       double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, int* nmod_used)
+{
+    const IgemmLaunchCfg ic = igemm_cfg(icfg);
+    const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
+    const int64_t gp = geo.gp;
+    const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
+    if (gp % ic.bm != 0 || gp % ic.bn != 0)
+        return fail(ctx, HSGEN_ERR_ARG, "column padding %lld is not a multiple of the INT8 tile", (long long)gp);
+    int nmod = nmod_req > 0 ? nmod_req : oz_moduli_for(target, kd, ng);
+    if (nmod <= 1 || nmod > kMaxModuli)
+        return fail(ctx, HSGEN_ERR_ACCURACY, "no modulus count in [2, %d] reaches rel err %.3g at depth %lld",
+                    kMaxModuli, target, (long long)kd);
+    const OzPlan pl = oz_plan(nmod, kd);
+    if (pl.beta < 8)
+        return fail(ctx, HSGEN_ERR_ACCURACY, "%d moduli leave only %d bits per element", nmod, pl.beta);
+    *nmod_used = nmod;
+    // exponents: eZ, eZs, eW
+    hsgen_status st = ensure(ctx, ctx->oz_e, sizeof(int32_t) * 3 * (size_t)gp, false);
+    if (st) return st;
+    int32_t* eZ = (int32_t*)ctx->oz_e.p;
+    int32_t* eZs = eZ + gp;
+    int32_t* eW = eZs + gp;
+    const double2* Z = (const double2*)ctx->Z.p;
+    const double2* Zs = (const double2*)ctx->Zs.p;
+    const double2* W = (const double2*)ctx->W.p;
+    HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ctx->stream));
+    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ctx->stream));
+    if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ctx->stream));
+    // residue arrays of one modulus: Zr, Zrot, Zs_r, W_r (2 bytes per complex)
+    const int64_t ldo = 2 * kd;
+    const size_t arr = (size_t)ldo * gp;
+    st = ensure(ctx, ctx->oz_res, 4 * arr, true);   // zero: rows beyond ldz stay 0
+    if (st) return st;
+    int8_t* Zr = (int8_t*)ctx->oz_res.p;
+    int8_t* Zrot = Zr + arr;
+    int8_t* Zsr = Zr + 2 * arr;
+    int8_t* Wr = Zr + 3 * arr;
+    // double-double accumulators (hi, lo) for S and H, ld = ng
+    const size_t accn = (size_t)ng * ng;
+    st = ensure(ctx, ctx->oz_dd, sizeof(double2) * 4 * accn, false);
+    if (st) return st;
+    double2* Shi = (double2*)ctx->oz_dd.p;
+    double2* Slo = Shi + accn;
+    double2* Hhi = Shi + 2 * accn;
+    double2* Hlo = Shi + 3 * accn;
+    // tile list (shared by every modulus)
+    std::vector<IgemmProblem> probs;
+    std::vector<TileWork> work;
+    {
+        std::vector<GemmProblem> gp_dummy;
+        Geometry g2 = geo;
+        g2.t.bm = ic.bm;
+        g2.t.bn = ic.bn;
+        // same tile order as the FP64 path; problem ids: S (if any) then H
+        add_hs_problems(gp_dummy, work, g2, ng, nullptr, nullptr, nullptr, nullptr, nullptr, 0, flags);
+        if (do_s) {
+            IgemmProblem p{Zr, Zrot, Zsr, ldo, ldo, Shi, Slo, ng, ng, ng, (int32_t)kd, 1};
+            probs.push_back(p);
+        }
+        if (do_h) {
+            IgemmProblem p{Zr, Zrot, Wr, ldo, ldo, Hhi, Hlo, ng, ng, ng, (int32_t)kd, 1};
+            probs.push_back(p);
+        }
+        xcd_permute(work);
+    }
+    st = ensure(ctx, ctx->oz_probs, sizeof(IgemmProblem) * probs.size(), false);
+    if (st) return st;
+    st = ensure(ctx, ctx->oz_work, sizeof(TileWork) * work.size(), false);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(ctx->oz_probs.p, probs.data(), sizeof(IgemmProblem) * probs.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->oz_work.p, work.data(), sizeof(TileWork) * work.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    const int rows = (int)std::min<int64_t>(geo.ldz, kd);
+    for (int m = 0; m < nmod; ++m) {
+        OzResParams rp;
+        std::memset(&rp, 0, sizeof rp);
+        rp.ldx = geo.ldz;
+        rp.rows = (int32_t)kd;
+        rp.rows_src = rows;
+        rp.beta = pl.beta;
+        rp.nmod = 1;
+        rp.p[0] = kModuli[m];
+        rp.invp[0] = 1.0 / kModuli[m];
+        rp.ldo = ldo;
+        rp.X = Z;
+        rp.e = eZ;
+        rp.out[0] = Zr;
+        rp.out_rot[0] = Zrot;
+        HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
+        rp.out_rot[0] = nullptr;
+        if (do_s) {
+            rp.X = Zs;
+            rp.e = eZs;
+            rp.out[0] = Zsr;
+            HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
+        }
+        if (do_h) {
+            rp.X = W;
+            rp.e = eW;
+            rp.out[0] = Wr;
+            HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
+        }
+        IgemmModulus md;
+        md.p = kModuli[m];
+        md.first = m == 0;
+        md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
+        md.pad = 0;
+        md.invp = 1.0 / kModuli[m];
+        md.whi = pl.whi[m];
+        md.wlo = pl.wlo[m];
+        HIPCHK(launch_igemm(icfg, (const IgemmProblem*)ctx->oz_probs.p, (const TileWork*)ctx->oz_work.p,
+                            (int)work.size(), md, ctx->stream));
+    }
+    OzFinalParams fp;
+    fp.ldacc = ng;
+    fp.beta = pl.beta;
+    fp.n = ng;
+    fp.Phi = pl.Phi;
+    fp.Plo = pl.Plo;
+    if (do_s) {
+        fp.Chi = Shi;
+        fp.Clo = Slo;
+        fp.eL = eZ;
+        fp.eR = eZs;
+        fp.C = dS;
+        fp.ldc = ldS;
+        HIPCHK(launch_crt_final(fp, ctx->stream));
+    }
+    if (do_h) {
+        fp.Chi = Hhi;
+        fp.Clo = Hlo;
+        fp.eL = eZ;
+        fp.eR = eW;
+        fp.C = dH;
+        fp.ldc = ldH;
+        HIPCHK(launch_crt_final(fp, ctx->stream));
+    }
+    // the host vectors above must outlive the async copies
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return HSGEN_OK;
+}
+
 // Common tail of create_hs / contract_hs: Z, Zs resident -> tmul -> S, H.
-hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng, int flags,
+hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng, const hsgen_opts& o,
                        double* H, double* S, int64_t ld, hsgen_report* rep, double ms_front)
 {
+    const int flags = o.flags;
     hsgen_status st;
     double2* Z = (double2*)ctx->Z.p;
     double2* Zs = (double2*)ctx->Zs.p;
@@ -664,16 +893,23 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     probs.clear();
     work.clear();
-    add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
-    xcd_permute(work);
-    for (auto& p : probs) {
-        p.ldl = geo.ldz;
-        p.ldr = geo.ldz;
-        p.kdepth = (int32_t)geo.kpad;
-        p.ldc = (p.C == dH) ? ldH : ldS;
+    int nmod_used = 0, icfg = 0;
+    if (o.precision == HSGEN_PREC_OZAKI_INT8) {
+        icfg = (o.tile > 0 && igemm_cfg(o.tile).bm > 0) ? o.tile : kIgemmCfg;
+        st = run_ozaki(ctx, geo, ng, flags, icfg, o.slices, o.target_rel_err, dH, ldH, dS, ldS, &nmod_used);
+        if (st) return st;
+    } else {
+        add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
+        xcd_permute(work);
+        for (auto& p : probs) {
+            p.ldl = geo.ldz;
+            p.ldr = geo.ldz;
+            p.kdepth = (int32_t)geo.kpad;
+            p.ldc = (p.C == dH) ? ldH : ldS;
+        }
+        st = run_grouped(ctx, geo.cfg, probs, work);
+        if (st) return st;
     }
-    st = run_grouped(ctx, geo.cfg, probs, work);
-    if (st) return st;
     if (flags & HSGEN_FILL_UPPER) {
         if (need_h) HIPCHK(launch_fill_upper(dH, ldH, ng, ctx->stream));
         if (!(flags & HSGEN_SKIP_S)) HIPCHK(launch_fill_upper(dS, ldS, ng, ctx->stream));
@@ -698,8 +934,8 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         rep->ng = ng;
         rep->kdim = s->kdim;
         rep->flops_alg = 16.0 * (double)ng * ng * s->kdim + 32.0 * (double)ng * (double)s->sum_nlm2;
-        rep->slices_used = 0;
-        rep->tile_used = geo.cfg;
+        rep->slices_used = nmod_used;
+        rep->tile_used = o.precision == HSGEN_PREC_OZAKI_INT8 ? icfg : geo.cfg;
     }
     return HSGEN_OK;
 }
@@ -753,8 +989,10 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (st) return st;
     hsgen_opts o{};
     if (opts) o = *opts;
-    if (o.precision != HSGEN_PREC_FP64)
-        return fail(ctx, HSGEN_ERR_UNSUPPORTED, "precision %d not available in this entry point", o.precision);
+    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
+        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    if (o.slices < 0 || o.slices > kMaxModuli)
+        return fail(ctx, HSGEN_ERR_ARG, "slices %d outside [0, %d]", o.slices, kMaxModuli);
     const int ng = kp->ng;
     if (ld < std::max(ng, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld %lld < ng %d", (long long)ld, ng);
     if ((!H && !(o.flags & HSGEN_SKIP_H)) || (!S && !(o.flags & HSGEN_SKIP_S)))
@@ -762,7 +1000,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (rep) std::memset(rep, 0, sizeof *rep);
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    const Geometry geo = geometry(o.tile, s->kdim, ng);
+    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng);
     st = upload_g(ctx, kp);
     if (st) return st;
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
@@ -778,7 +1016,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_abgen(p, true, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    st = finish_hs(ctx, s, geo, ng, o.flags, H, S, ld, rep, 0.0);
+    st = finish_hs(ctx, s, geo, ng, o, H, S, ld, rep, 0.0);
     if (st) return st;
     if (rep) {
         float t0 = 0;
@@ -819,8 +1057,10 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
         return fail(ctx, HSGEN_ERR_ARG, "max_lmax %d inconsistent with L %d", max_lmax, L);
     hsgen_opts o{};
     if (opts) o = *opts;
-    if (o.precision != HSGEN_PREC_FP64)
-        return fail(ctx, HSGEN_ERR_UNSUPPORTED, "precision %d not available in this entry point", o.precision);
+    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
+        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    if (o.slices < 0 || o.slices > kMaxModuli)
+        return fail(ctx, HSGEN_ERR_ARG, "slices %d outside [0, %d]", o.slices, kMaxModuli);
     if (ld < std::max(G, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld < G");
     if (rep) std::memset(rep, 0, sizeof *rep);
     HIPCHK(hipSetDevice(ctx->device));
@@ -853,7 +1093,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     }
     s.kdim = off;
     if (G == 0) return HSGEN_OK;
-    const Geometry geo = geometry(o.tile, s.kdim, G);
+    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G);
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);
     if (st) return st;
     struct Guard {
@@ -906,7 +1146,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_pack_ref(pp, (int)geo.gp, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    st = finish_hs(ctx, &s, geo, G, o.flags, H, S, ld, rep, 0.0);
+    st = finish_hs(ctx, &s, geo, G, o, H, S, ld, rep, 0.0);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (st) return st;
     if (rep) {

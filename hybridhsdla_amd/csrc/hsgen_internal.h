@@ -96,4 +96,65 @@ hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, 
                         hipStream_t st);
 hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st);
 
+// ---- split precision (ozaki.hip) ----
+constexpr int kMaxModuli = 20;
+
+struct OzResParams {
+    const double2* X;           // FP64 operand, column-major
+    int64_t ldx;
+    const int32_t* e;           // per-column exponents
+    int32_t rows;               // rows written (INT8 contraction depth)
+    int32_t rows_src;           // rows present in X; rows_src..rows-1 are written as 0
+    int32_t beta;
+    int32_t nmod;
+    int32_t p[kMaxModuli];
+    double invp[kMaxModuli];
+    int8_t* out[kMaxModuli];     // (re, im) residue pairs, column stride ldo bytes
+    int8_t* out_rot[kMaxModuli]; // (-im, re) pairs or null
+    int64_t ldo;
+};
+
+struct IgemmProblem {
+    const int8_t* Lre;
+    const int8_t* Lrot;
+    const int8_t* R;
+    int64_t ldl, ldr;           // bytes
+    double2* Chi;               // double-double accumulator, complex (re, im) parts
+    double2* Clo;
+    int64_t ldc;                // complex units
+    int32_t m, n, kdepth, lower;
+};
+
+struct IgemmModulus {
+    int32_t p;
+    int32_t first;
+    int32_t chunk_stages;
+    int32_t pad;
+    double invp;
+    double whi, wlo;            // u / p as double-double
+};
+
+struct IgemmLaunchCfg {
+    int bm, bn, bk, threads, lds_bytes;
+};
+
+struct OzFinalParams {
+    const double2* Chi;
+    const double2* Clo;
+    int64_t ldacc;
+    const int32_t* eL;
+    const int32_t* eR;
+    int32_t beta, n;
+    double Phi, Plo;            // product of the moduli as double-double
+    double2* C;
+    int64_t ldc;
+};
+
+hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, hipStream_t st);
+hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);
+IgemmLaunchCfg igemm_cfg(int id);
+hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
+                        const IgemmModulus& md, hipStream_t st);
+hipError_t launch_crt_final(const OzFinalParams& p, hipStream_t st);
+
 }  // namespace hsgen

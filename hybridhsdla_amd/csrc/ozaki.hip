@@ -1,0 +1,383 @@
+// Split-precision contraction on INT8 MFMA (Ozaki scheme II: integer
+// residues modulo small pairwise-coprime moduli, exact INT32 products,
+// Chinese-remainder recombination in FP64).
+//
+// For C = L^H R (complex, depth kd):
+//   1. per column j of each FP64 operand X: e_j with max_k |X_kj| < 2^e_j
+//      (re and im), X'_kj = rint(X_kj 2^(beta - e_j)), |X'| <= 2^beta;
+//   2. for each modulus p_m (<= 255): the symmetric residues X' mod p_m fit
+//      INT8; L is stored twice, as (Lr, Li) and as (-Li, Lr) pairs, so that
+//        Re C' = sum (Lr Rr + Li Ri),   Im C' = sum (-Li Rr + Lr Ri)
+//      are each one INT8 dot product over the interleaved pairs;
+//   3. v_mfma_i32_16x16x64_i8 accumulates exactly in INT32; the accumulator is
+//      reduced mod p_m every <= 2080 MFMAs (|products| <= 127^2) so it never
+//      overflows;
+//   4. the epilogue adds r_m * (u_m / p_m) (u_m = (P/p_m)^-1 mod p_m, P the
+//      product of the moduli) to a double-double accumulator per element; the
+//      sum is C'/P modulo 1, exact up to the double-double rounding;
+//   5. the final pass takes the centred fractional part, multiplies by P and
+//      rescales by 2^(e_i + e_j - 2 beta).
+// P > 2 max|C'| = 2 * 2kd * 2^(2 beta) is chosen by the host, which sets beta
+// from the number of moduli.
+#include "hsgen_internal.h"
+
+namespace hsgen {
+
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const i4v* gi4ptr;
+
+// ---------------------------------------------------------------------------
+// 1. column exponents
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_colexp(const double2* __restrict__ X, int64_t ldx, int rows,
+                                                int ncols, int32_t* __restrict__ e)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.x;
+    if (col >= ncols) return;
+    const double2* c = X + (int64_t)col * ldx;
+    double m = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        double2 v = c[r];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int ex = 0;
+        if (red[0] > 0.0) (void)frexp(red[0], &ex);   // red[0] < 2^ex
+        e[col] = ex;
+    }
+}
+
+hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_colexp, dim3(ncols), dim3(256), 0, st, X, ldx, rows, ncols, e);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 2. residues
+// ---------------------------------------------------------------------------
+// symmetric residue of an integer-valued double |x| <= 2^53 modulo p
+__device__ inline int sym_mod(double x, double p, double invp, int lo, int hi)
+{
+    double q = rint(x * invp);
+    int r = (int)fma(-q, p, x);     // exact: |q p| < 2^53
+    if (r > hi) r -= (int)p;
+    if (r < lo) r += (int)p;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_residues(OzResParams p)
+{
+    const int col = blockIdx.y;
+    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r0 >= p.rows) return;
+    const int sh = p.beta - p.e[col];
+    const double2 v = r0 < p.rows_src ? p.X[(int64_t)col * p.ldx + r0] : make_double2(0.0, 0.0);
+    const double xr = rint(ldexp(v.x, sh));
+    const double xi = rint(ldexp(v.y, sh));
+    for (int m = 0; m < p.nmod; ++m) {
+        const double pm = (double)p.p[m];
+        const int hi = (p.p[m] - 1) / 2;
+        const int lo = -(p.p[m] / 2);
+        const int rr = sym_mod(xr, pm, p.invp[m], lo, hi);
+        const int ri = sym_mod(xi, pm, p.invp[m], lo, hi);
+        char2* o = (char2*)(p.out[m] + (int64_t)col * p.ldo) + r0;
+        *o = make_char2((signed char)rr, (signed char)ri);
+        if (p.out_rot[m]) {
+            char2* orot = (char2*)(p.out_rot[m] + (int64_t)col * p.ldo) + r0;
+            *orot = make_char2((signed char)(-ri), (signed char)rr);
+        }
+    }
+}
+
+hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st)
+{
+    dim3 grid((p.rows + 255) / 256, ncols), block(256);
+    hipLaunchKernelGGL(k_residues, grid, block, 0, st, p);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 3. INT8 MFMA GEMM with CRT epilogue
+// ---------------------------------------------------------------------------
+// double-double helpers
+__device__ inline void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ inline void fast_two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    e = b - (s - a);
+}
+
+// (hi, lo) <- frac((hi, lo) + r * (whi, wlo)), |frac| <= 1/2
+__device__ inline void dd_accum(double& hi, double& lo, double r, double whi, double wlo, bool first)
+{
+    double ph = r * whi;
+    double pl = fma(r, whi, -ph);
+    pl = fma(r, wlo, pl);
+    double s, e;
+    if (first) {
+        s = ph;
+        e = pl;
+    } else {
+        two_sum(hi, ph, s, e);
+        e += lo + pl;
+    }
+    fast_two_sum(s, e, s, e);
+    const double n = rint(s);
+    s -= n;                              // exact
+    two_sum(s, e, hi, lo);               // s may now be smaller than e
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN>
+struct IgemmCfg {
+    static constexpr int kThreads = WGM * WGN * 64;
+    static constexpr int kWM = BM / WGM;
+    static constexpr int kWN = BN / WGN;
+    static constexpr int kRB = kWM / 16;
+    static constexpr int kCB = kWN / 16;
+    static constexpr int kColBytes = 2 * BK;                  // bytes per operand column per stage
+    static constexpr int kColStride = kColBytes + 16;         // + pad: conflict-free 16-lane reads
+    static constexpr int kCols = 2 * BM + BN;                 // Lre, Lrot, R columns
+    static constexpr int kStageBytes = kCols * kColStride;
+    static constexpr int kChunks = kCols * (kColBytes / 16);  // 16-byte loads per stage
+    static constexpr int kLoads = kChunks / kThreads;
+    static_assert(BK % 32 == 0, "BK must be a multiple of one MFMA k-step (32 complex)");
+    static_assert(kChunks % kThreads == 0, "stage must split evenly over threads");
+};
+
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM * WGN * 64) void k_igemm_crt(const IgemmProblem* __restrict__ probs,
+                                                              const TileWork* __restrict__ work, int nwork,
+                                                              IgemmModulus md)
+{
+    using Cfg = IgemmCfg<BM, BN, BK, WGM, WGN>;
+    constexpr int NT = Cfg::kThreads;
+    constexpr int RB = Cfg::kRB, CB = Cfg::kCB;
+    constexpr int CS = Cfg::kColStride;
+    constexpr int NL = Cfg::kLoads;
+    constexpr int CPC = Cfg::kColBytes / 16;     // 16-byte chunks per column
+    extern __shared__ __attribute__((aligned(16))) char ilds[];
+
+    const int wid = blockIdx.x;
+    if (wid >= nwork) return;
+    const TileWork tw = work[wid];
+    const IgemmProblem pb = probs[tw.prob];
+    const int I0 = tw.bi * BM;
+    const int J0 = tw.bj * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave % WGM;
+    const int wn = wave / WGM;
+
+    gi4ptr src[NL];
+    int ldsoff[NL];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+        const int e = tid + NT * r;
+        const int c = e / CPC;
+        const int q = e - c * CPC;
+        const int8_t* p;
+        if (c < BM) p = pb.Lre + (int64_t)(I0 + c) * pb.ldl + 16 * q;
+        else if (c < 2 * BM) p = pb.Lrot + (int64_t)(I0 + c - BM) * pb.ldl + 16 * q;
+        else p = pb.R + (int64_t)(J0 + c - 2 * BM) * pb.ldr + 16 * q;
+        src[r] = (gi4ptr)(const void*)p;
+        ldsoff[r] = c * CS + 16 * q;
+    }
+
+    i4v are[RB][CB], aim[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            are[i][j] = i4v{0, 0, 0, 0};
+            aim[i][j] = i4v{0, 0, 0, 0};
+        }
+
+    const int nst = pb.kdepth / BK;
+    const int stride_q = 2 * BK / 16;   // stage advance in 16-byte units
+    i4v stage[NL];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) stage[r] = src[r][0];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) *(i4v*)&ilds[ldsoff[r]] = stage[r];
+    __syncthreads();
+
+    const int offA = (wm * Cfg::kWM + (lane & 15)) * CS + 16 * (lane >> 4);
+    const int offAr = (BM + wm * Cfg::kWM + (lane & 15)) * CS + 16 * (lane >> 4);
+    const int offB = (2 * BM + wn * Cfg::kWN + (lane & 15)) * CS + 16 * (lane >> 4);
+    const double pm = (double)md.p;
+    const double invp = md.invp;
+
+    for (int s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        const int64_t qnext = (int64_t)min(s + 1, nst - 1) * stride_q;
+#pragma unroll
+        for (int r = 0; r < NL; ++r) stage[r] = src[r][qnext];
+        __builtin_amdgcn_sched_barrier(0);
+        const char* lb = ilds + buf * Cfg::kStageBytes;
+#pragma unroll
+        for (int ks = 0; ks < BK / 32; ++ks) {
+            i4v a[RB], ar[RB], b[CB];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                a[i] = *(const i4v*)&lb[offA + i * 16 * CS + 64 * ks];
+                ar[i] = *(const i4v*)&lb[offAr + i * 16 * CS + 64 * ks];
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) b[j] = *(const i4v*)&lb[offB + j * 16 * CS + 64 * ks];
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    aim[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        char* nb = ilds + (buf ^ 1) * Cfg::kStageBytes;
+#pragma unroll
+        for (int r = 0; r < NL; ++r) *(i4v*)&nb[ldsoff[r]] = stage[r];
+        // keep |acc| below 2^31: reduce mod p every chunk_stages stages
+        if ((s + 1) % md.chunk_stages == 0 && s + 1 < nst) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double x = (double)are[i][j][q];
+                        are[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                        x = (double)aim[i][j][q];
+                        aim[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                    }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: residue -> double-double CRT accumulation.
+    // lane holds D[4*(lane>>4) + q][lane & 15] in register q
+    const bool lower = pb.lower != 0;
+    const bool first = md.first != 0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int gj = J0 + wn * Cfg::kWN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = I0 + wm * Cfg::kWM + i * 16 + 4 * (lane >> 4) + q;
+                if (gi < pb.m && gj < pb.n && (!lower || gi >= gj)) {
+                    double xr = (double)are[i][j][q];
+                    double xi = (double)aim[i][j][q];
+                    xr = fma(-floor(xr * invp), pm, xr);      // [0, p)
+                    xi = fma(-floor(xi * invp), pm, xi);
+                    const int64_t idx = (int64_t)gj * pb.ldc + gi;
+                    double2 h = first ? make_double2(0.0, 0.0) : pb.Chi[idx];
+                    double2 l = first ? make_double2(0.0, 0.0) : pb.Clo[idx];
+                    dd_accum(h.x, l.x, xr, md.whi, md.wlo, first);
+                    dd_accum(h.y, l.y, xi, md.whi, md.wlo, first);
+                    pb.Chi[idx] = h;
+                    pb.Clo[idx] = l;
+                }
+            }
+        }
+}
+
+#define HSGEN_IGEMM_CFGS(X)        \
+    X(1, 128, 128, 64, 2, 2)        \
+    X(2, 128, 128, 32, 2, 2)        \
+    X(3, 64, 64, 64, 2, 2)          \
+    X(4, 128, 128, 64, 2, 4)        \
+    X(5, 128, 128, 32, 2, 4)        \
+    X(6, 128, 64, 64, 2, 2)
+
+IgemmLaunchCfg igemm_cfg(int id)
+{
+#define X(ID, BM, BN, BK, WGM, WGN) \
+    if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2};
+    HSGEN_IGEMM_CFGS(X)
+#undef X
+    return IgemmLaunchCfg{0, 0, 0, 0, 0};
+}
+
+hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
+                        const IgemmModulus& md, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+#define X(ID, BM, BN, BK, WGM, WGN)                                                            \
+    if (id == ID) {                                                                            \
+        constexpr int lds = IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2;                  \
+        static bool attr_set = false;                                                          \
+        if (!attr_set) {                                                                       \
+            hipError_t e = hipFuncSetAttribute((const void*)k_igemm_crt<BM, BN, BK, WGM, WGN>,  \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+            if (e != hipSuccess) return e;                                                     \
+            attr_set = true;                                                                   \
+        }                                                                                      \
+        hipLaunchKernelGGL((k_igemm_crt<BM, BN, BK, WGM, WGN>), dim3(nwork), dim3(WGM * WGN * 64), \
+                           lds, st, probs, work, nwork, md);                                   \
+        return hipGetLastError();                                                              \
+    }
+    HSGEN_IGEMM_CFGS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// 5. CRT finalisation: C = 2^(eL_i + eR_j - 2 beta) * P * centred_frac(acc)
+// ---------------------------------------------------------------------------
+__global__ void k_crt_final(OzFinalParams p)
+{
+    const int i = blockIdx.x * 16 + threadIdx.x;
+    const int j = blockIdx.y * 16 + threadIdx.y;
+    if (i >= p.n || j >= p.n || i < j) return;
+    const int64_t a = (int64_t)j * p.ldacc + i;
+    const double2 h = p.Chi[a];
+    const double2 l = p.Clo[a];
+    const int sh = p.eL[i] + p.eR[j] - 2 * p.beta;
+    double out[2];
+    const double hh[2] = {h.x, h.y}, ll[2] = {l.x, l.y};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        // accumulator already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
+        double fh = hh[c], fl = ll[c];
+        const double n = rint(fh);
+        fh -= n;
+        double s, e;
+        two_sum(fh, fl, s, e);
+        // (s + e) * (Phi + Plo)
+        double v = s * p.Phi;
+        double err = fma(s, p.Phi, -v);
+        err = fma(s, p.Plo, err);
+        err = fma(e, p.Phi, err);
+        out[c] = ldexp(v + err, sh);
+    }
+    if (i == j) out[1] = 0.0;
+    p.C[(int64_t)j * p.ldc + i] = make_double2(out[0], out[1]);
+}
+
+hipError_t launch_crt_final(const OzFinalParams& p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    dim3 grid((p.n + 15) / 16, (p.n + 15) / 16), block(16, 16);
+    hipLaunchKernelGGL(k_crt_final, grid, block, 0, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace hsgen

@@ -101,3 +101,35 @@ def test_tile_configs_agree(ctx):
         H2, S2 = ctx.create_HS2(h, kp, tile=t)
         assert rel_fro_lower(H2, H1) <= 1e-14
         assert rel_fro_lower(S2, S1) <= 1e-14
+
+
+# ---------------------------------------------------------------------------
+# split precision (INT8 MFMA, Ozaki scheme II)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("target", [1e-10, 1e-12, 1e-13])
+def test_split_precision_meets_target(ctx, target):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kps = make_config(0)
+    kp = kps[0]
+    h = ctx.upload(s)
+    H, S, rep = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=target,
+                               report=True)
+    Hr, Sr = po.reference_hs(s, kp)
+    eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
+    print(f"target {target:.0e}: moduli {rep['slices_used']} rel err H {eh:.3e} S {es:.3e}", flush=True)
+    assert eh <= target and es <= target
+    assert not np.any(np.triu(H, 1)) and not np.any(np.triu(S, 1))
+
+
+def test_split_precision_error_falls_with_moduli(ctx):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=3, lmax=(3, 4, 2), ng_target=150)
+    h = ctx.upload(s)
+    H64, S64 = ctx.create_HS2(h, kp)
+    errs = []
+    for n in (8, 10, 12, 14, 16):
+        H, S = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=n)
+        errs.append(max(rel_fro_lower(H, H64), rel_fro_lower(S, S64)))
+    print("errors by moduli 8..16:", errs, flush=True)
+    assert all(b < a for a, b in zip(errs, errs[1:]))
+    assert errs[-1] <= 1e-14
