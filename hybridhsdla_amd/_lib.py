@@ -65,6 +65,8 @@ class HsgenReport(C.Structure):
         ("kdim", C.c_int32),
         ("slices_used", C.c_int32),
         ("tile_used", C.c_int32),
+        ("err_est_h", C.c_double),
+        ("err_est_s", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -152,7 +152,7 @@ struct Geometry {
     int64_t gp;     // padded column count
 };
 
-Geometry geometry(int tile, int kdim, int ng)
+Geometry geometry(int tile, int kdim, int ng, int col_align = 1)
 {
     Geometry g;
     g.cfg = tile > 0 ? tile : kDefaultTile;
@@ -161,7 +161,7 @@ Geometry geometry(int tile, int kdim, int ng)
         g.cfg = kDefaultTile;
         g.t = zgemm_cfg(g.cfg);
     }
-    const int64_t tm = std::max(g.t.bm, g.t.bn);
+    const int64_t tm = std::max({g.t.bm, g.t.bn, col_align});
     g.kpad = roundup(2 * (int64_t)kdim, g.t.bk);
     g.ldz = g.kpad + g.t.bk;
     g.gp = roundup(std::max(ng, 1), tm);
@@ -656,16 +656,15 @@ int oz_beta(int n, int64_t kd)
     return (int)std::floor((l2p - 2.0 - std::log2(2.0 * (double)kd)) / 2.0);
 }
 
-// Number of moduli for a relative error target.  Rounding the scaled
-// operands to beta bits perturbs each element of C by ~2^-beta |L_i| |R_j|,
-// so the relative Frobenius error is ~c 2^-beta with c growing like the
-// square root of the matrix dimension (H cancels more than S, its T being
-// indefinite); three bits of margin on top.
+// First guess of the modulus count for a relative error target: the error is
+// c 2^-beta with c data dependent (~0.1 to ~50 on the systems measured: H,
+// whose T is indefinite, cancels more than S); the guess takes c = 1 and the
+// a-posteriori estimate in run_ozaki decides whether more moduli are needed.
 int oz_moduli_for(double target, int64_t kd, int ng)
 {
+    (void)ng;
     if (!(target > 0.0)) target = 1e-12;
-    const double dim = (double)std::max<int64_t>(kd, ng);
-    const int need = (int)std::ceil(-std::log2(target) + 0.5 * std::log2(dim) + 3.0);
+    const int need = (int)std::ceil(-std::log2(target));
     for (int n = 2; n <= kMaxModuli; ++n)
         if (oz_beta(n, kd) >= need) return n;
     return -1;
@@ -704,7 +703,8 @@ OzPlan oz_plan(int nmod, int64_t kd)
 
 // S = Z^H Zs and H = Z^H W from the resident FP64 operands, on INT8 MFMA.
 hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod_req,
-                       double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, int* nmod_used)
+                       double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, int* nmod_used,
+                       double* est_h, double* est_s)
 {
     const IgemmLaunchCfg ic = igemm_cfg(icfg);
     const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
@@ -720,27 +720,24 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     if (pl.beta < 8)
         return fail(ctx, HSGEN_ERR_ACCURACY, "%d moduli leave only %d bits per element", nmod, pl.beta);
     *nmod_used = nmod;
-    // exponents: eZ, eZs, eW
-    hsgen_status st = ensure(ctx, ctx->oz_e, sizeof(int32_t) * 3 * (size_t)gp, false);
+    // per-column exponents eZ, eZs, eW and squared column norms (error estimate)
+    const int kNblk = 1024;
+    hsgen_status st = ensure(ctx, ctx->oz_e, (sizeof(int32_t) + sizeof(double)) * 3 * (size_t)gp +
+                                                  sizeof(double) * 2 * kNblk, false);
     if (st) return st;
-    int32_t* eZ = (int32_t*)ctx->oz_e.p;
+    double* ssZ = (double*)ctx->oz_e.p;
+    double* ssZs = ssZ + gp;
+    double* ssW = ssZs + gp;
+    double* part = ssW + gp;                       // 2 x kNblk partial sums of |C|^2
+    int32_t* eZ = (int32_t*)(part + 2 * kNblk);
     int32_t* eZs = eZ + gp;
     int32_t* eW = eZs + gp;
     const double2* Z = (const double2*)ctx->Z.p;
     const double2* Zs = (const double2*)ctx->Zs.p;
     const double2* W = (const double2*)ctx->W.p;
-    HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ctx->stream));
-    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ctx->stream));
-    if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ctx->stream));
-    // residue arrays of one modulus: Zr, Zrot, Zs_r, W_r (2 bytes per complex)
-    const int64_t ldo = 2 * kd;
-    const size_t arr = (size_t)ldo * gp;
-    st = ensure(ctx, ctx->oz_res, 4 * arr, true);   // zero: rows beyond ldz stay 0
-    if (st) return st;
-    int8_t* Zr = (int8_t*)ctx->oz_res.p;
-    int8_t* Zrot = Zr + arr;
-    int8_t* Zsr = Zr + 2 * arr;
-    int8_t* Wr = Zr + 3 * arr;
+    HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ssZ, ctx->stream));
+    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
+    if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ssW, ctx->stream));
     // double-double accumulators (hi, lo) for S and H, ld = ng
     const size_t accn = (size_t)ng * ng;
     st = ensure(ctx, ctx->oz_dd, sizeof(double2) * 4 * accn, false);
@@ -749,6 +746,26 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     double2* Slo = Shi + accn;
     double2* Hhi = Shi + 2 * accn;
     double2* Hlo = Shi + 3 * accn;
+    // Residue arrays of one modulus ("slot"): Zr, Zs_r, W_r (+ rotated copies
+    // the kernel reads), 2 bytes per complex.  As many moduli as fit in 60% of
+    // the free memory share one residue pass, which reads the FP64 operands
+    // once for the whole group.
+    const int64_t ldo = 2 * kd;
+    const size_t arr = (size_t)ldo * gp;
+    const int narr = ic.needs_rot == 1 ? 4 : ic.needs_rot == 2 ? 5 : 3;
+    const size_t slot_bytes = narr * arr;
+    int group = 1;
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+            fr += ctx->oz_res.bytes;   // the current residue buffer may be regrown
+            group = (int)std::max<size_t>(1, std::min<size_t>((size_t)nmod, (size_t)(0.6 * fr) / slot_bytes));
+        }
+        group = std::min(group, 8);
+    }
+    st = ensure(ctx, ctx->oz_res, group * slot_bytes, false);
+    if (st) return st;
+    auto slot = [&](int i, int a) -> int8_t* { return (int8_t*)ctx->oz_res.p + i * slot_bytes + a * arr; };
     // tile list (shared by every modulus)
     std::vector<IgemmProblem> probs;
     std::vector<TileWork> work;
@@ -759,15 +776,36 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         g2.t.bn = ic.bn;
         // same tile order as the FP64 path; problem ids: S (if any) then H
         add_hs_problems(gp_dummy, work, g2, ng, nullptr, nullptr, nullptr, nullptr, nullptr, 0, flags);
-        if (do_s) {
-            IgemmProblem p{Zr, Zrot, Zsr, ldo, ldo, Shi, Slo, ng, ng, ng, (int32_t)kd, 1};
-            probs.push_back(p);
-        }
-        if (do_h) {
-            IgemmProblem p{Zr, Zrot, Wr, ldo, ldo, Hhi, Hlo, ng, ng, ng, (int32_t)kd, 1};
-            probs.push_back(p);
-        }
         xcd_permute(work);
+    }
+    // Problem sets, one per (slot, depth chunk).  Kernels without an in-kernel
+    // mod reduction get the depth in INT32-safe chunks, one launch each: every
+    // chunk adds its own residue to the CRT sum (the sum is only needed modulo
+    // 1, so the chunks need not be combined modulo p first).
+    const int64_t kc = ic.max_kchunk > 0 ? ic.max_kchunk : kd;
+    const int nchunk = (int)((kd + kc - 1) / kc);
+    size_t nprob = 0;
+    for (int i = 0; i < group; ++i) {
+        int8_t* Zr = slot(i, 0);
+        int8_t* Zsr = slot(i, 1);
+        int8_t* Wr = slot(i, 2);
+        int8_t* Zrot = ic.needs_rot == 1 ? slot(i, 3) : nullptr;     // (-Li, Lr) of Z
+        int8_t* Zsrot = ic.needs_rot == 2 ? slot(i, 3) : nullptr;    // (Ri, -Rr) of Zs
+        int8_t* Wrot = ic.needs_rot == 2 ? slot(i, 4) : nullptr;     // (Ri, -Rr) of W
+        std::vector<IgemmProblem> base;
+        if (do_s) base.push_back(IgemmProblem{Zr, Zrot, Zsr, Zsrot, ldo, ldo, Shi, Slo, ng, ng, ng, (int32_t)kd, 1});
+        if (do_h) base.push_back(IgemmProblem{Zr, Zrot, Wr, Wrot, ldo, ldo, Hhi, Hlo, ng, ng, ng, (int32_t)kd, 1});
+        nprob = base.size();
+        for (int c = 0; c < nchunk; ++c)
+            for (IgemmProblem p : base) {
+                const int64_t off = 2 * (int64_t)c * kc;   // bytes
+                p.Lre += off;
+                if (p.Lrot) p.Lrot += off;
+                p.R += off;
+                if (p.Rrot) p.Rrot += off;
+                p.kdepth = (int32_t)std::min<int64_t>(kc, kd - (int64_t)c * kc);
+                probs.push_back(p);
+            }
     }
     st = ensure(ctx, ctx->oz_probs, sizeof(IgemmProblem) * probs.size(), false);
     if (st) return st;
@@ -778,45 +816,59 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     HIPCHK(hipMemcpyAsync(ctx->oz_work.p, work.data(), sizeof(TileWork) * work.size(),
                           hipMemcpyHostToDevice, ctx->stream));
     const int rows = (int)std::min<int64_t>(geo.ldz, kd);
-    for (int m = 0; m < nmod; ++m) {
+    for (int m0 = 0; m0 < nmod; m0 += group) {
+        const int gn = std::min(group, nmod - m0);
         OzResParams rp;
         std::memset(&rp, 0, sizeof rp);
         rp.ldx = geo.ldz;
         rp.rows = (int32_t)kd;
         rp.rows_src = rows;
         rp.beta = pl.beta;
-        rp.nmod = 1;
-        rp.p[0] = kModuli[m];
-        rp.invp[0] = 1.0 / kModuli[m];
+        rp.nmod = gn;
         rp.ldo = ldo;
+        for (int i = 0; i < gn; ++i) {
+            rp.p[i] = kModuli[m0 + i];
+            rp.invp[i] = 1.0 / kModuli[m0 + i];
+        }
+        auto outputs = [&](int a, int arot) {
+            for (int i = 0; i < gn; ++i) {
+                rp.out[i] = slot(i, a);
+                rp.out_rot[i] = arot >= 0 ? slot(i, arot) : nullptr;
+            }
+        };
         rp.X = Z;
         rp.e = eZ;
-        rp.out[0] = Zr;
-        rp.out_rot[0] = Zrot;
+        rp.rot_kind = 1;
+        outputs(0, ic.needs_rot == 1 ? 3 : -1);
         HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
-        rp.out_rot[0] = nullptr;
+        rp.rot_kind = 2;
         if (do_s) {
             rp.X = Zs;
             rp.e = eZs;
-            rp.out[0] = Zsr;
+            outputs(1, ic.needs_rot == 2 ? 3 : -1);
             HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
         }
         if (do_h) {
             rp.X = W;
             rp.e = eW;
-            rp.out[0] = Wr;
+            outputs(2, ic.needs_rot == 2 ? 4 : -1);
             HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
         }
-        IgemmModulus md;
-        md.p = kModuli[m];
-        md.first = m == 0;
-        md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
-        md.pad = 0;
-        md.invp = 1.0 / kModuli[m];
-        md.whi = pl.whi[m];
-        md.wlo = pl.wlo[m];
-        HIPCHK(launch_igemm(icfg, (const IgemmProblem*)ctx->oz_probs.p, (const TileWork*)ctx->oz_work.p,
-                            (int)work.size(), md, ctx->stream));
+        for (int i = 0; i < gn; ++i) {
+            const int m = m0 + i;
+            for (int c = 0; c < nchunk; ++c) {
+                IgemmModulus md;
+                md.p = kModuli[m];
+                md.first = m == 0 && c == 0;
+                md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
+                md.pad = 0;
+                md.invp = 1.0 / kModuli[m];
+                md.whi = pl.whi[m];
+                md.wlo = pl.wlo[m];
+                HIPCHK(launch_igemm(icfg, (const IgemmProblem*)ctx->oz_probs.p + (i * nchunk + c) * nprob,
+                                    (const TileWork*)ctx->oz_work.p, (int)work.size(), md, ctx->stream));
+            }
+        }
     }
     OzFinalParams fp;
     fp.ldacc = ng;
@@ -842,8 +894,38 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         fp.ldc = ldH;
         HIPCHK(launch_crt_final(fp, ctx->stream));
     }
+    // Error estimate.  Rounding an operand element to beta bits leaves an
+    // independent error uniform within +-2^(e_col - beta - 1) in each real
+    // part, so E|dC_ij|^2 = 2^-2beta / 6 (2^(2 eL_i) |R_j|^2 + 2^(2 eR_j) |L_i|^2);
+    // summed over the lower triangle (about half of all i, j) and divided by
+    // the computed |tril(C)|_F.
+    if (do_s) HIPCHK(launch_tril_sumsq(dS, ldS, ng, part, kNblk, ctx->stream));
+    if (do_h) HIPCHK(launch_tril_sumsq(dH, ldH, ng, part + kNblk, kNblk, ctx->stream));
+    std::vector<double> hss(3 * (size_t)gp + 2 * kNblk);
+    std::vector<int32_t> he(3 * (size_t)gp);
+    HIPCHK(hipMemcpyAsync(hss.data(), ssZ, sizeof(double) * hss.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(he.data(), eZ, sizeof(int32_t) * he.size(), hipMemcpyDeviceToHost, ctx->stream));
     // the host vectors above must outlive the async copies
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto estimate = [&](int which) {   // 0: S (R = Zs), 1: H (R = W)
+        const double* sl = hss.data();
+        const double* sr = hss.data() + (which == 0 ? 1 : 2) * gp;
+        const int32_t* el = he.data();
+        const int32_t* er = he.data() + (which == 0 ? 1 : 2) * gp;
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+        for (int i = 0; i < ng; ++i) {
+            a += std::ldexp(1.0, 2 * (el[i] - pl.beta));
+            b += sr[i];
+            c += std::ldexp(1.0, 2 * (er[i] - pl.beta));
+            d += sl[i];
+        }
+        double n2 = 0.0;
+        for (int k = 0; k < kNblk; ++k) n2 += hss[3 * (size_t)gp + which * kNblk + k];
+        const double e2 = (a * b + c * d) / 12.0;
+        return n2 > 0.0 ? std::sqrt(e2 / n2) : 0.0;
+    };
+    *est_s = do_s ? estimate(0) : 0.0;
+    *est_h = do_h ? estimate(1) : 0.0;
     return HSGEN_OK;
 }
 
@@ -894,10 +976,23 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
     probs.clear();
     work.clear();
     int nmod_used = 0, icfg = 0;
+    double est_h = 0.0, est_s = 0.0;
     if (o.precision == HSGEN_PREC_OZAKI_INT8) {
         icfg = (o.tile > 0 && igemm_cfg(o.tile).bm > 0) ? o.tile : kIgemmCfg;
-        st = run_ozaki(ctx, geo, ng, flags, icfg, o.slices, o.target_rel_err, dH, ldH, dS, ldS, &nmod_used);
+        st = run_ozaki(ctx, geo, ng, flags, icfg, o.slices, o.target_rel_err, dH, ldH, dS, ldS, &nmod_used,
+                       &est_h, &est_s);
         if (st) return st;
+        // automatic modulus count: check the estimate and redo with enough extra
+        // moduli (each adds ~4 bits to beta) when the target is missed
+        const double target = o.target_rel_err > 0.0 ? o.target_rel_err : 1e-12;
+        const double worst = std::max(est_h, est_s);
+        if (o.slices == 0 && worst > target && nmod_used < kMaxModuli) {
+            const int extra = (int)std::ceil((std::log2(worst / target) + 1.0) / 3.9);
+            const int n2 = std::min(kMaxModuli, nmod_used + std::max(extra, 1));
+            st = run_ozaki(ctx, geo, ng, flags, icfg, n2, target, dH, ldH, dS, ldS, &nmod_used, &est_h,
+                           &est_s);
+            if (st) return st;
+        }
     } else {
         add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
         xcd_permute(work);
@@ -935,6 +1030,8 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         rep->kdim = s->kdim;
         rep->flops_alg = 16.0 * (double)ng * ng * s->kdim + 32.0 * (double)ng * (double)s->sum_nlm2;
         rep->slices_used = nmod_used;
+        rep->err_est_h = est_h;
+        rep->err_est_s = est_s;
         rep->tile_used = o.precision == HSGEN_PREC_OZAKI_INT8 ? icfg : geo.cfg;
     }
     return HSGEN_OK;
@@ -1000,7 +1097,8 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (rep) std::memset(rep, 0, sizeof *rep);
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng);
+    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
+                                  o.precision == HSGEN_PREC_FP64 ? 1 : 256);
     st = upload_g(ctx, kp);
     if (st) return st;
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
@@ -1093,7 +1191,8 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     }
     s.kdim = off;
     if (G == 0) return HSGEN_OK;
-    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G);
+    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
+                                  o.precision == HSGEN_PREC_FP64 ? 1 : 256);
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);
     if (st) return st;
     struct Guard {

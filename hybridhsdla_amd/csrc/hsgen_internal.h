@@ -110,14 +110,16 @@ struct OzResParams {
     int32_t p[kMaxModuli];
     double invp[kMaxModuli];
     int8_t* out[kMaxModuli];     // (re, im) residue pairs, column stride ldo bytes
-    int8_t* out_rot[kMaxModuli]; // (-im, re) pairs or null
+    int8_t* out_rot[kMaxModuli]; // rotated pairs or null
+    int32_t rot_kind;            // 1: (-im, re) (left operand), 2: (im, -re) (right operand)
     int64_t ldo;
 };
 
 struct IgemmProblem {
-    const int8_t* Lre;
-    const int8_t* Lrot;
-    const int8_t* R;
+    const int8_t* Lre;          // (Lr, Li) residue pairs
+    const int8_t* Lrot;         // (-Li, Lr) pairs (kernels that rotate L), else null
+    const int8_t* R;            // (Rr, Ri) pairs
+    const int8_t* Rrot;         // (Ri, -Rr) pairs (kernels that rotate R), else null
     int64_t ldl, ldr;           // bytes
     double2* Chi;               // double-double accumulator, complex (re, im) parts
     double2* Clo;
@@ -136,6 +138,8 @@ struct IgemmModulus {
 
 struct IgemmLaunchCfg {
     int bm, bn, bk, threads, lds_bytes;
+    int needs_rot;              // 1: reads precomputed (-Li, Lr); 2: reads precomputed (Ri, -Rr)
+    int max_kchunk;             // > 0: no in-kernel mod reduction, launch at most this depth (complex)
 };
 
 struct OzFinalParams {
@@ -150,7 +154,9 @@ struct OzFinalParams {
     int64_t ldc;
 };
 
-hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, hipStream_t st);
+hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, double* ss,
+                         hipStream_t st);
+hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st);
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);
 IgemmLaunchCfg igemm_cfg(int id);
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,

@@ -29,34 +29,69 @@ typedef __attribute__((address_space(1))) const i4v* gi4ptr;
 // ---------------------------------------------------------------------------
 // 1. column exponents
 // ---------------------------------------------------------------------------
+// also the column's squared 2-norm (for the error estimate)
 __global__ __launch_bounds__(256) void k_colexp(const double2* __restrict__ X, int64_t ldx, int rows,
-                                                int ncols, int32_t* __restrict__ e)
+                                                int ncols, int32_t* __restrict__ e, double* __restrict__ ss)
 {
-    __shared__ double red[256];
+    __shared__ double red[256], red2[256];
     const int col = blockIdx.x;
     if (col >= ncols) return;
     const double2* c = X + (int64_t)col * ldx;
-    double m = 0.0;
+    double m = 0.0, q = 0.0;
     for (int r = threadIdx.x; r < rows; r += blockDim.x) {
         double2 v = c[r];
         m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+        q = fma(v.x, v.x, fma(v.y, v.y, q));
     }
     red[threadIdx.x] = m;
+    red2[threadIdx.x] = q;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+            red2[threadIdx.x] += red2[threadIdx.x + s];
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         int ex = 0;
         if (red[0] > 0.0) (void)frexp(red[0], &ex);   // red[0] < 2^ex
         e[col] = ex;
+        ss[col] = red2[0];
     }
 }
 
-hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, hipStream_t st)
+hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, double* ss,
+                         hipStream_t st)
 {
-    hipLaunchKernelGGL(k_colexp, dim3(ncols), dim3(256), 0, st, X, ldx, rows, ncols, e);
+    hipLaunchKernelGGL(k_colexp, dim3(ncols), dim3(256), 0, st, X, ldx, rows, ncols, e, ss);
+    return hipGetLastError();
+}
+
+// Partial sums of |C_ij|^2 over the lower triangle (i >= j), one per block;
+// grid (nblk), each block strides over columns.
+__global__ __launch_bounds__(256) void k_tril_sumsq(const double2* __restrict__ C, int64_t ldc, int n,
+                                                    double* __restrict__ part)
+{
+    __shared__ double red[256];
+    double q = 0.0;
+    for (int j = blockIdx.x; j < n; j += gridDim.x)
+        for (int i = j + threadIdx.x; i < n; i += blockDim.x) {
+            const double2 v = C[(int64_t)j * ldc + i];
+            q = fma(v.x, v.x, fma(v.y, v.y, q));
+        }
+    red[threadIdx.x] = q;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tril_sumsq, dim3(nblk), dim3(256), 0, st, C, ldc, n, part);
     return hipGetLastError();
 }
 
@@ -92,7 +127,8 @@ __global__ __launch_bounds__(256) void k_residues(OzResParams p)
         *o = make_char2((signed char)rr, (signed char)ri);
         if (p.out_rot[m]) {
             char2* orot = (char2*)(p.out_rot[m] + (int64_t)col * p.ldo) + r0;
-            *orot = make_char2((signed char)(-ri), (signed char)rr);
+            *orot = p.rot_kind == 1 ? make_char2((signed char)(-ri), (signed char)rr)
+                                    : make_char2((signed char)ri, (signed char)(-rr));
         }
     }
 }
@@ -299,6 +335,230 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm_crt(const IgemmProblem
         }
 }
 
+// ---------------------------------------------------------------------------
+// 3b. INT8 MFMA GEMM, pipelined: operands go straight from global memory into
+// an NS-deep LDS ring (global_load_lds_dwordx4 writes lane l's 16 bytes at
+// LDS base + 16 l), so NS-1 stages of loads are in flight while a stage is
+// multiplied.  The (-Li, Lr) operand for the imaginary part is formed in
+// registers from the (Lr, Li) fragment, so L is read once.
+//
+// LDS stage layout: column c (L columns first, then R) holds KS complex = 2 KS
+// bytes = CPC chunks of 16 bytes; chunk q of column c sits at position
+// q ^ f(c), f(c) = (c / (16 / CPC)) % CPC, which makes the 16-lane fragment
+// reads (16 consecutive columns, one chunk) hit 16 distinct bank groups.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR>
+struct Igemm2Cfg {
+    static constexpr int kThreads = WGM * WGN * 64;
+    static constexpr int kWaves = WGM * WGN;
+    static constexpr int kWM = BM / WGM;
+    static constexpr int kWN = BN / WGN;
+    static constexpr int kRB = kWM / 16;
+    static constexpr int kCB = kWN / 16;
+    static constexpr int kCPC = 2 * KS / 16;                     // 16-byte chunks per column
+    static constexpr int kColBytes = 2 * KS;
+    static constexpr int kCols = BM + ((ROTR & 1) ? 2 : 1) * BN;  // L, R [, Rrot] columns
+    static constexpr int kStageBytes = kCols * kColBytes;
+    static constexpr int kWaveLoads = kCols * kCPC / 64;          // 1 KiB wave loads per stage
+    static constexpr int kLoadsPerWave = kWaveLoads / kWaves;
+    static constexpr int kLdsBytes = NS * kStageBytes;
+    static_assert(KS == 32 || KS == 64, "stage depth: one or two MFMA k-steps");
+    static_assert(kWaveLoads % kWaves == 0, "stage must split evenly over waves");
+    static_assert(kLoadsPerWave * (NS - 2) <= 63, "vmcnt range");
+};
+
+// s_waitcnt immediate that waits for vmcnt <= N and nothing else
+// (vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15)
+template <int N>
+__device__ constexpr int vmcnt_only()
+{
+    return (N & 15) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14);
+}
+
+template <int CPC>
+__device__ inline int chunk_swz(int c)
+{
+    return (c / (16 / CPC)) % CPC;
+}
+
+// (Lr, Li) byte pairs -> (-Li, Lr) byte pairs (mod 256 negation, no borrow
+// between the two 16-bit halves)
+__device__ inline int rot_pairs(int x)
+{
+    const unsigned u = (unsigned)x;
+    const unsigned hi = (u << 8) & 0xFF00FF00u;                       // Lr -> odd bytes
+    const unsigned lo = (0x01000100u - ((u >> 8) & 0x00FF00FFu)) & 0x00FF00FFu;   // -Li -> even bytes
+    return (int)(hi | lo);
+}
+
+template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR>
+__global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProblem* __restrict__ probs,
+                                                               const TileWork* __restrict__ work, int nwork,
+                                                               IgemmModulus md)
+{
+    using Cfg = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>;
+    constexpr int RB = Cfg::kRB, CB = Cfg::kCB;
+    constexpr int CPC = Cfg::kCPC;
+    constexpr int LPW = Cfg::kLoadsPerWave;
+    constexpr int SB = Cfg::kStageBytes;
+    extern __shared__ __attribute__((aligned(1024))) char ilds[];
+
+    const int wid = blockIdx.x;
+    if (wid >= nwork) return;
+    const TileWork tw = work[wid];
+    const IgemmProblem pb = probs[tw.prob];
+    const int I0 = tw.bi * BM;
+    const int J0 = tw.bj * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % WGM;
+    const int wn = wave / WGM;
+
+    // this lane's source for each of the wave's loads of a stage (stage 0)
+    const int8_t* src[LPW];
+#pragma unroll
+    for (int r = 0; r < LPW; ++r) {
+        const int slot = (wave * LPW + r) * 64 + lane;
+        const int c = slot / CPC;
+        const int q = (slot % CPC) ^ chunk_swz<CPC>(c);
+        if (c < BM) src[r] = pb.Lre + (int64_t)(I0 + c) * pb.ldl + 16 * q;
+        else if (c < BM + BN) src[r] = pb.R + (int64_t)(J0 + c - BM) * pb.ldr + 16 * q;
+        else src[r] = pb.Rrot + (int64_t)(J0 + c - BM - BN) * pb.ldr + 16 * q;
+    }
+    typedef __attribute__((address_space(3))) char lchar;
+    lchar* lbase = (lchar*)ilds;
+
+    auto issue = [&](int stage_k, int buf) {
+#pragma unroll
+        for (int r = 0; r < LPW; ++r) {
+            const __attribute__((address_space(1))) void* g =
+                (const __attribute__((address_space(1))) void*)(src[r] + (int64_t)stage_k * (2 * KS));
+            __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(lbase + buf * SB +
+                                                                                           (wave * LPW + r) * 1024),
+                                             16, 0, 0);
+        }
+    };
+
+    i4v are[RB][CB], aim[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            are[i][j] = i4v{0, 0, 0, 0};
+            aim[i][j] = i4v{0, 0, 0, 0};
+        }
+
+    const int nst = pb.kdepth / KS;
+    // prologue: stages 0 .. NS-2 in flight
+#pragma unroll
+    for (int p = 0; p < NS - 1; ++p) issue(min(p, nst - 1), p);
+
+    // fragment read offsets (bytes within a stage) for k-step 0; chunk of k-step ks is (lane>>4) + 4 ks
+    int offA[RB], offB[CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        const int c = wm * Cfg::kWM + i * 16 + (lane & 15);
+        offA[i] = c * Cfg::kColBytes;
+    }
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        const int c = BM + wn * Cfg::kWN + j * 16 + (lane & 15);
+        offB[j] = c * Cfg::kColBytes;
+    }
+    const int q0 = lane >> 4;
+    const double pm = (double)md.p;
+    const double invp = md.invp;
+
+    for (int s = 0; s < nst; ++s) {
+        // stage s landed (NS-2 younger stages may still be in flight), and
+        // every wave is done with stage s-1, whose buffer is refilled next
+        // (__syncthreads would wait for every LDS-DMA load in flight, i.e. drain
+        // the prefetch ring; each wave instead waits for its own loads of stage
+        // s and the bare barrier publishes them.  The previous stage's LDS reads
+        // completed before their MFMAs issued.)
+        __builtin_amdgcn_s_waitcnt(vmcnt_only<LPW * (NS - 2)>());
+        asm volatile("s_barrier" ::: "memory");
+        issue(min(s + NS - 1, nst - 1), (s + NS - 1) % NS);
+        const char* lb = ilds + (s % NS) * SB;
+#pragma unroll
+        for (int ks = 0; ks < KS / 32; ++ks) {
+            i4v a[RB], ar[RB], b[CB], br[CB];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const int c = wm * Cfg::kWM + i * 16 + (lane & 15);
+                a[i] = *(const i4v*)&lb[offA[i] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int c = BM + wn * Cfg::kWN + j * 16 + (lane & 15);
+                b[j] = *(const i4v*)&lb[offB[j] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
+                if (ROTR & 1) {
+                    const int cr = c + BN;
+                    br[j] = *(const i4v*)&lb[offB[j] + BN * Cfg::kColBytes + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(cr))];
+                }
+            }
+            if (!(ROTR & 1)) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    ar[i] = i4v{rot_pairs(a[i][0]), rot_pairs(a[i][1]), rot_pairs(a[i][2]), rot_pairs(a[i][3])};
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    aim[i][j] = (ROTR & 1) ? __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], br[j], aim[i][j], 0, 0, 0)
+                                           : __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
+        }
+        // ROTR & 2: the host keeps kdepth within one INT32-safe chunk
+        if (!(ROTR & 2) && (s + 1) % md.chunk_stages == 0 && s + 1 < nst) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double x = (double)are[i][j][q];
+                        are[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                        x = (double)aim[i][j][q];
+                        aim[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                    }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+
+    const bool lower = pb.lower != 0;
+    const bool first = md.first != 0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int gj = J0 + wn * Cfg::kWN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = I0 + wm * Cfg::kWM + i * 16 + 4 * (lane >> 4) + q;
+                if (gi < pb.m && gj < pb.n && (!lower || gi >= gj)) {
+                    double xr = (double)are[i][j][q];
+                    double xi = (double)aim[i][j][q];
+                    xr = fma(-floor(xr * invp), pm, xr);
+                    xi = fma(-floor(xi * invp), pm, xi);
+                    const int64_t idx = (int64_t)gj * pb.ldc + gi;
+                    double2 h = first ? make_double2(0.0, 0.0) : pb.Chi[idx];
+                    double2 l = first ? make_double2(0.0, 0.0) : pb.Clo[idx];
+                    dd_accum(h.x, l.x, xr, md.whi, md.wlo, first);
+                    dd_accum(h.y, l.y, xi, md.whi, md.wlo, first);
+                    pb.Chi[idx] = h;
+                    pb.Clo[idx] = l;
+                }
+            }
+        }
+}
+
 #define HSGEN_IGEMM_CFGS(X)        \
     X(1, 128, 128, 64, 2, 2)        \
     X(2, 128, 128, 32, 2, 2)        \
@@ -307,13 +567,41 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm_crt(const IgemmProblem
     X(5, 128, 128, 32, 2, 4)        \
     X(6, 128, 64, 64, 2, 2)
 
+// pipelined kernels: (id, BM, BN, KS, WGM, WGN, NS, ROTR)
+#define HSGEN_IGEMM2_CFGS(Y)            \
+    Y(11, 256, 128, 64, 4, 2, 3, 0)      \
+    Y(12, 256, 128, 32, 4, 2, 4, 0)      \
+    Y(13, 128, 128, 64, 2, 2, 4, 0)      \
+    Y(14, 128, 128, 64, 2, 4, 4, 0)      \
+    Y(15, 128, 256, 64, 2, 4, 3, 0)      \
+    Y(16, 256, 128, 32, 4, 2, 5, 0)      \
+    Y(21, 256, 128, 32, 4, 2, 4, 1)      \
+    Y(22, 256, 128, 32, 4, 2, 3, 1)      \
+    Y(23, 128, 128, 64, 2, 4, 3, 1)      \
+    Y(24, 256, 128, 64, 4, 2, 2, 1)      \
+    Y(25, 128, 256, 32, 2, 4, 3, 1)      \
+    Y(26, 128, 256, 64, 2, 2, 3, 0)      \
+    Y(27, 128, 256, 32, 2, 2, 4, 0)      \
+    Y(28, 256, 128, 64, 4, 1, 3, 0)      \
+    Y(29, 256, 256, 32, 4, 2, 3, 0)      \
+    Y(31, 128, 256, 64, 2, 2, 3, 2)      \
+    Y(32, 128, 256, 32, 2, 2, 4, 2)      \
+    Y(33, 256, 128, 64, 4, 1, 3, 2)      \
+    Y(34, 256, 128, 64, 4, 2, 3, 2)      \
+    Y(35, 256, 128, 32, 4, 2, 4, 2)
+
 IgemmLaunchCfg igemm_cfg(int id)
 {
 #define X(ID, BM, BN, BK, WGM, WGN) \
-    if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2};
+    if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2, 1, 0};
     HSGEN_IGEMM_CFGS(X)
 #undef X
-    return IgemmLaunchCfg{0, 0, 0, 0, 0};
+#define Y(ID, BM, BN, KS, WGM, WGN, NS, ROTR) \
+    if (id == ID) return IgemmLaunchCfg{BM, BN, KS, WGM * WGN * 64, Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>::kLdsBytes, \
+                                        (ROTR & 1) ? 2 : 0, (ROTR & 2) ? 65536 : 0};
+    HSGEN_IGEMM2_CFGS(Y)
+#undef Y
+    return IgemmLaunchCfg{0, 0, 0, 0, 0, 0, 0};
 }
 
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
@@ -336,6 +624,23 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
     }
     HSGEN_IGEMM_CFGS(X)
 #undef X
+#define Y(ID, BM, BN, KS, WGM, WGN, NS, ROTR)                                                   \
+    if (id == ID) {                                                                              \
+        constexpr int lds = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>::kLdsBytes;                \
+        static bool attr_set = false;                                                            \
+        if (!attr_set) {                                                                         \
+            hipError_t e = hipFuncSetAttribute(                                                  \
+                (const void*)k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, ROTR>,                       \
+                hipFuncAttributeMaxDynamicSharedMemorySize, lds);                                \
+            if (e != hipSuccess) return e;                                                       \
+            attr_set = true;                                                                     \
+        }                                                                                        \
+        hipLaunchKernelGGL((k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, ROTR>), dim3(nwork),           \
+                           dim3(WGM * WGN * 64), lds, st, probs, work, nwork, md);               \
+        return hipGetLastError();                                                                \
+    }
+    HSGEN_IGEMM2_CFGS(Y)
+#undef Y
     return hipErrorInvalidValue;
 }
 

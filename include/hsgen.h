@@ -88,7 +88,9 @@ typedef struct {
 
 typedef struct {
     int32_t precision;         /* hsgen_precision */
-    int32_t slices;            /* split precision: slices per operand (0 = from target_rel_err) */
+    int32_t slices;            /* split precision: INT8 residue slices per operand, i.e. the
+                                  number of moduli (0 = chosen for target_rel_err and checked
+                                  afterwards against the error estimate, with a retry) */
     double target_rel_err;     /* split precision: target relative Frobenius error (0 = 1e-12) */
     int32_t flags;             /* HSGEN_FILL_UPPER | HSGEN_SKIP_H | HSGEN_SKIP_S */
     int32_t tile;              /* contraction tile config id (0 = autotuned default) */
@@ -106,6 +108,8 @@ typedef struct {
     int32_t kdim;              /* K = sum_a L_a */
     int32_t slices_used;       /* split precision: slices actually used */
     int32_t tile_used;         /* contraction tile config id actually used */
+    double err_est_h;          /* split precision: estimated relative Frobenius error of H */
+    double err_est_s;          /* ... and of S (0 in FP64) */
 } hsgen_report;
 
 /* Context: bind GPU `device` (HIP ordinal), create its stream.

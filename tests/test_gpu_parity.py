@@ -133,3 +133,15 @@ def test_split_precision_error_falls_with_moduli(ctx):
     print("errors by moduli 8..16:", errs, flush=True)
     assert all(b < a for a, b in zip(errs, errs[1:]))
     assert errs[-1] <= 1e-14
+
+
+def test_split_precision_tile_configs_bitwise_equal(ctx):
+    """Every INT8 kernel configuration computes the same exact integer
+    products and the same recombination, so the results agree bit for bit."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=3, lmax=(3, 4, 2), ng_target=300)
+    h = ctx.upload(s)
+    H0, S0 = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=13, tile=1)
+    for t in (2, 3, 4, 5, 6, 11, 12, 13, 14, 15, 16, 21, 22, 23, 24, 25, 26, 27, 28, 29, 31, 32, 33, 34, 35):
+        H, S = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=13, tile=t)
+        assert np.array_equal(H, H0) and np.array_equal(S, S0), t
