@@ -123,6 +123,72 @@ def sampled_parity(system, kp, H_dev, S_dev, ld, nsample=48, seed=1):
     return {"rel_err_H": rel(Hs, Hr), "rel_err_S": rel(Ss, Sr), "n_sample": int(idx.size)}
 
 
+def lower_rel_err(X, ref, rows=2048):
+    """||tril(X - ref)||_F / ||tril(ref)||_F for column-major device matrices
+    held in torch tensors (torch row r = matrix column r, so the matrix's lower
+    triangle is the tensor's upper triangle), in row blocks."""
+    import torch
+    num = 0.0
+    den = 0.0
+    n = ref.shape[0]
+    for r0 in range(0, n, rows):
+        r1 = min(n, r0 + rows)
+        d = torch.triu(X[r0:r1] - ref[r0:r1], diagonal=r0)
+        num += float(torch.sum(d.real ** 2 + d.imag ** 2))
+        rr = torch.triu(ref[r0:r1], diagonal=r0)
+        den += float(torch.sum(rr.real ** 2 + rr.imag ** 2))
+    return (num / den) ** 0.5 if den > 0 else 0.0
+
+
+def split_run(args, system, G, H, S, step, fp64_ms, F):
+    """Split precision (INT8 Ozaki) on the same k-point at the stated error
+    target, timed like the FP64 run, with its error against the FP64 result."""
+    import numpy as np
+    import torch
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+
+    H64 = H.clone()
+    S64 = S.clone()
+    for _ in range(args.warmup):
+        step(HSGEN_PREC_OZAKI_INT8, args.split_target)
+    torch.cuda.synchronize()
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reps.append(step(HSGEN_PREC_OZAKI_INT8, args.split_target))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    eH = lower_rel_err(H, H64)
+    eS = lower_rel_err(S, S64)
+    del H64, S64
+    r = reps[-1]
+    nmod = int(r["slices_used"])
+    ms_ig = float(np.mean([x["ms_igemm"] for x in reps]))
+    ops = nmod * 16.0 * G * G * system.kdim      # INT8 ops of the nmod modulus GEMMs
+    ach = ops / (ms_ig * 1e-3) / 1e12 if ms_ig > 0 else None
+    return {
+        "sec_per_hs": round(ms * 1e-3, 4),
+        "value": round(F / (ms * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s (FP64-equivalent)",
+        "speedup_vs_fp64": round(fp64_ms / ms, 3),
+        "target_rel_err": args.split_target,
+        "moduli": nmod,
+        "rel_err_H_vs_fp64": eH,
+        "rel_err_S_vs_fp64": eS,
+        "err_model_H": r["err_est_h"], "err_model_S": r["err_est_s"],
+        "err_sample_H": r["err_smp_h"], "err_sample_S": r["err_smp_s"],
+        "breakdown_ms": {"contract": round(float(np.mean([x["ms_contract"] for x in reps])), 3),
+                         "int8_gemm": round(ms_ig, 3)},
+        "roofline": {
+            "bound": "mfma", "kernel": "k_igemm2_crt (INT8 residue GEMMs, one launch per modulus)",
+            "achieved": round(ach, 1) if ach else None, "peak": INT8_PEAK, "unit": "TOP/s",
+            "frac": round(ach / INT8_PEAK, 4) if ach else None,
+            "algorithmic_ops_per_launch": 16.0 * G * G * system.kdim,
+            "avg_launch_ms": round(ms_ig / max(nmod, 1), 3),
+        },
+    }
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM traffic per launch from the committed rocprofv3 --pmc summary."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_traffic.json"
@@ -146,6 +212,11 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split-precision comparison run after the FP64 run")
+    ap.add_argument("--split-target", type=float, default=1e-12)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="GiB: run one device-to-device copy of this size first (traffic calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -172,10 +243,19 @@ def main(argv=None) -> int:
     H = torch.zeros((G, G), dtype=torch.complex128, device=dev)
     S = torch.zeros((G, G), dtype=torch.complex128, device=dev)
     prec = HSGEN_PREC_FP64 if args.precision == "fp64" else HSGEN_PREC_OZAKI_INT8
+    if args.copy_probe > 0:
+        n = int(args.copy_probe * 2**30) // 8
+        a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {n * 8} bytes read, {n * 8} bytes written")
+        del a, b
 
-    def step():
-        return ctx.create_hs_device(sysh, kp, H.data_ptr(), S.data_ptr(), G, precision=prec,
-                                    slices=args.slices, tile=args.tile)
+    def step(precision=prec, target=0.0):
+        return ctx.create_hs_device(sysh, kp, H.data_ptr(), S.data_ptr(), G, precision=precision,
+                                    slices=args.slices, target_rel_err=target, tile=args.tile)
 
     for i in range(args.warmup):
         r = step()
@@ -229,6 +309,14 @@ def main(argv=None) -> int:
         except Exception as e:  # the check must not hide the measurement
             check = {"error": str(e)}
 
+    split = None
+    if prec == HSGEN_PREC_FP64 and not args.no_split:
+        try:
+            split = split_run(args, system, G, H, S, step, ms_step, F)
+            log(f"[bench] split precision: {split}")
+        except Exception as e:
+            split = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -262,6 +350,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "abgen_roofline": abgen,
         "parity_sample": check,
+        "split_precision": split,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -67,6 +67,9 @@ class HsgenReport(C.Structure):
         ("tile_used", C.c_int32),
         ("err_est_h", C.c_double),
         ("err_est_s", C.c_double),
+        ("ms_igemm", C.c_double),
+        ("err_smp_h", C.c_double),
+        ("err_smp_s", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -93,6 +96,10 @@ SIGNATURES = [
      [_p, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, C.POINTER(_p), C.POINTER(_p),
       C.POINTER(_p), _p, C.c_int32, _p, C.POINTER(HsgenOpts), _p, _p, C.c_int64,
       C.POINTER(HsgenReport)]),
+    ("hsgen_radial_solve", C.c_int,
+     [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, _p, _p, _p, _p]),
+    ("hsgen_quadrature_weights", C.c_int, [C.c_int32, _p]),
+    ("hsgen_tmat_assemble", C.c_int, [C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p]),
 ]
 
 _lib = None

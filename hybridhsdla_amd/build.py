@@ -25,6 +25,9 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liborc.so"
 
 HIP_SOURCES = ["abgen.hip", "zgemm.hip", "ozaki.hip", "api.hip"]
+# host-only set-up code (radial solver, T assembly), linked into the same library
+HOST_SOURCES = ["radial.cpp", "tmat.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -72,11 +75,41 @@ def build_lib(jobs: int = 4) -> Path:
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         if len(procs) >= jobs:
             _drain(procs)
+    for src in HOST_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        objs.append(obj)
+        cmd = [shutil.which("g++") or "g++", *HOST_FLAGS, "-I", str(ROOT / "include"), "-c",
+               str(CSRC / src), "-o", str(obj)]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        if len(procs) >= jobs:
+            _drain(procs)
     _drain(procs)
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
     return LIB
+
+
+HDF5_ROOT = Path(os.environ.get("HSGEN_HDF5_ROOT", "/opt/conda"))
+IO_LIB = LIBDIR / "libhsgen_io.so"
+
+
+def build_io() -> Path | None:
+    """libhsgen_io.so: the Fleur dump reader / H-S writer (host C++, HDF5),
+    linked against libhsgen.so.  Skipped when no HDF5 is installed."""
+    inc, lib = HDF5_ROOT / "include", HDF5_ROOT / "lib"
+    if not (inc / "hdf5.h").exists() or not list(lib.glob("libhdf5.so*")):
+        print(f"HDF5 not found under {HDF5_ROOT}: libhsgen_io.so not built", file=sys.stderr)
+        return None
+    tmp = IO_LIB.with_suffix(".so.tmp")
+    _run([shutil.which("g++") or "g++", *HOST_FLAGS, "-shared", "-I", str(ROOT / "include"), "-isystem", str(inc),
+          "-o", str(tmp), str(CSRC / "dump_io.cpp"), "-L", str(LIBDIR), "-lhsgen", "-L", str(lib), "-lhdf5",
+          # RUNPATH (not RPATH) and a static C++ runtime: the HDF5 directory is
+          # searched for this library's own dependencies only, so its older
+          # libstdc++ is never picked up for the HIP runtime
+          "-static-libstdc++", "-static-libgcc", "-Wl,--enable-new-dtags", f"-Wl,-rpath,$ORIGIN:{lib}"])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 def _drain(procs) -> None:
@@ -106,6 +139,9 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if not args.oracle_only:
         print("built", build_lib())
+        io = build_io()
+        if io:
+            print("built", io)
     if not args.lib_only:
         print("built", build_oracle())
     return 0

@@ -41,7 +41,8 @@ struct hsgen_ctx {
     // workspace
     DevBuf Z, Zs, W, Htmp, Stmp, A, B, g, probs, work, unorm2, atoms_tmp;
     // split precision
-    DevBuf oz_e, oz_res, oz_dd, oz_probs, oz_work;
+    DevBuf oz_e, oz_res, oz_dd, oz_probs, oz_work, oz_smp;
+    hipEvent_t oz_ev[2 * kMaxModuli] = {};   // INT8 GEMM start/stop per residue group
 };
 
 struct hsgen_sys {
@@ -355,6 +356,7 @@ hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
+    for (auto& ev : ctx->oz_ev) HIPCHK(hipEventCreate(&ev));
     *out = ctx;
     return HSGEN_OK;
 }
@@ -366,7 +368,7 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
                       &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp, &ctx->oz_e, &ctx->oz_res,
-                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work})
+                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp})
         release(*b);
     return HSGEN_OK;
 }
@@ -378,6 +380,8 @@ hsgen_status hsgen_finalize(hsgen_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     hsgen_release_workspace(ctx);
     for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : ctx->oz_ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -629,7 +633,7 @@ hsgen_status copy_lower_to_host(hsgen_ctx* ctx, const double2* dsrc, int n, doub
 // Pairwise-coprime moduli <= 255 (symmetric residues fit INT8), largest first.
 const int kModuli[kMaxModuli] = {255, 254, 253, 251, 247, 241, 239, 233, 229, 227,
                                  223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
-const int kIgemmCfg = 1;
+const int kIgemmCfg = 11;
 
 struct OzPlan {
     int nmod = 0;
@@ -648,33 +652,30 @@ int inv_mod(int a, int m)
     return 0;
 }
 
-// beta for n moduli and depth kd: P > 2 * (2 kd) * 2^(2 beta)
-int oz_beta(int n, int64_t kd)
+// beta for n moduli: the largest beta with P >= 2^(2 beta + 1.01).  The
+// integer products are bounded by Cauchy-Schwarz, |C'_ij| <= ||L'_i|| ||R'_j||
+// < 2^(2 beta) (1 + 2^-19), whatever the depth; beta <= 52 keeps every scaled
+// element an exact double.
+int oz_beta(int n)
 {
     double l2p = 0.0;
     for (int m = 0; m < n; ++m) l2p += std::log2((double)kModuli[m]);
-    return (int)std::floor((l2p - 2.0 - std::log2(2.0 * (double)kd)) / 2.0);
+    return std::min(52, (int)std::floor((l2p - 1.01) / 2.0));
 }
 
-// First guess of the modulus count for a relative error target: the error is
-// c 2^-beta with c data dependent (~0.1 to ~50 on the systems measured: H,
-// whose T is indefinite, cancels more than S); the guess takes c = 1 and the
-// a-posteriori estimate in run_ozaki decides whether more moduli are needed.
-int oz_moduli_for(double target, int64_t kd, int ng)
+// Smallest beta for which rounding the elements of a column of depth kd
+// (complex) keeps its norm below 2^beta (1 + 2^-20): the rounding adds at most
+// sqrt(2 kd) / 2 to a norm below 2^beta.
+int oz_beta_min(int64_t kd)
 {
-    (void)ng;
-    if (!(target > 0.0)) target = 1e-12;
-    const int need = (int)std::ceil(-std::log2(target));
-    for (int n = 2; n <= kMaxModuli; ++n)
-        if (oz_beta(n, kd) >= need) return n;
-    return -1;
+    return (int)std::ceil(19.0 + 0.5 * std::log2(2.0 * (double)kd));
 }
 
-OzPlan oz_plan(int nmod, int64_t kd)
+OzPlan oz_plan(int nmod)
 {
     OzPlan pl;
     pl.nmod = nmod;
-    pl.beta = oz_beta(nmod, kd);
+    pl.beta = oz_beta(nmod);
     // P as double-double (exact while the product fits 106 bits)
     double ph = 1.0, plo = 0.0;
     for (int m = 0; m < nmod; ++m) {
@@ -701,46 +702,48 @@ OzPlan oz_plan(int nmod, int64_t kd)
     return pl;
 }
 
-// S = Z^H Zs and H = Z^H W from the resident FP64 operands, on INT8 MFMA.
-hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod_req,
-                       double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, int* nmod_used,
-                       double* est_h, double* est_s)
+// Modelled relative Frobenius error of tril(C), C = L^H R, at beta.  Rounding
+// an element of column i to beta bits of the column's norm exponent e_i leaves
+// an independent error uniform within +-2^(e_i - beta - 1) in each real part,
+// so E|dC_ij|^2 = 2^-2beta / 6 (2^(2 eL_i) ||R_j||^2 + 2^(2 eR_j) ||L_i||^2);
+// the lower triangle holds about half of all (i, j).
+double oz_model(const int32_t* el, const double* sl, const int32_t* er, const double* sr, int ng, int beta,
+                double tril_norm2)
+{
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    for (int i = 0; i < ng; ++i) {
+        a += std::ldexp(1.0, 2 * (el[i] - beta));
+        b += sr[i];
+        c += std::ldexp(1.0, 2 * (er[i] - beta));
+        d += sl[i];
+    }
+    return tril_norm2 > 0.0 ? std::sqrt((a * b + c * d) / 12.0 / tril_norm2) : 0.0;
+}
+
+struct OzResult {
+    int nmod = 0;
+    double est_h = 0.0, est_s = 0.0;     // modelled error, exact norms
+    double smp_h = 0.0, smp_s = 0.0;     // measured error on the FP64 sample block
+    double ms_igemm = 0.0;
+};
+
+// One split-precision evaluation with nmod moduli: residues, INT8 GEMMs with
+// CRT accumulation, final recombination into dS / dH.
+hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod,
+                        const int32_t* eZ, const int32_t* eZs, const int32_t* eW, double2* dH,
+                        int64_t ldH, double2* dS, int64_t ldS, double* ms_igemm)
 {
     const IgemmLaunchCfg ic = igemm_cfg(icfg);
     const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
     const int64_t gp = geo.gp;
     const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
-    if (gp % ic.bm != 0 || gp % ic.bn != 0)
-        return fail(ctx, HSGEN_ERR_ARG, "column padding %lld is not a multiple of the INT8 tile", (long long)gp);
-    int nmod = nmod_req > 0 ? nmod_req : oz_moduli_for(target, kd, ng);
-    if (nmod <= 1 || nmod > kMaxModuli)
-        return fail(ctx, HSGEN_ERR_ACCURACY, "no modulus count in [2, %d] reaches rel err %.3g at depth %lld",
-                    kMaxModuli, target, (long long)kd);
-    const OzPlan pl = oz_plan(nmod, kd);
-    if (pl.beta < 8)
-        return fail(ctx, HSGEN_ERR_ACCURACY, "%d moduli leave only %d bits per element", nmod, pl.beta);
-    *nmod_used = nmod;
-    // per-column exponents eZ, eZs, eW and squared column norms (error estimate)
-    const int kNblk = 1024;
-    hsgen_status st = ensure(ctx, ctx->oz_e, (sizeof(int32_t) + sizeof(double)) * 3 * (size_t)gp +
-                                                  sizeof(double) * 2 * kNblk, false);
-    if (st) return st;
-    double* ssZ = (double*)ctx->oz_e.p;
-    double* ssZs = ssZ + gp;
-    double* ssW = ssZs + gp;
-    double* part = ssW + gp;                       // 2 x kNblk partial sums of |C|^2
-    int32_t* eZ = (int32_t*)(part + 2 * kNblk);
-    int32_t* eZs = eZ + gp;
-    int32_t* eW = eZs + gp;
+    const OzPlan pl = oz_plan(nmod);
     const double2* Z = (const double2*)ctx->Z.p;
     const double2* Zs = (const double2*)ctx->Zs.p;
     const double2* W = (const double2*)ctx->W.p;
-    HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ssZ, ctx->stream));
-    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
-    if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ssW, ctx->stream));
     // double-double accumulators (hi, lo) for S and H, ld = ng
     const size_t accn = (size_t)ng * ng;
-    st = ensure(ctx, ctx->oz_dd, sizeof(double2) * 4 * accn, false);
+    hsgen_status st = ensure(ctx, ctx->oz_dd, sizeof(double2) * 4 * accn, false);
     if (st) return st;
     double2* Shi = (double2*)ctx->oz_dd.p;
     double2* Slo = Shi + accn;
@@ -816,7 +819,8 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     HIPCHK(hipMemcpyAsync(ctx->oz_work.p, work.data(), sizeof(TileWork) * work.size(),
                           hipMemcpyHostToDevice, ctx->stream));
     const int rows = (int)std::min<int64_t>(geo.ldz, kd);
-    for (int m0 = 0; m0 < nmod; m0 += group) {
+    int ngroups = 0;
+    for (int m0 = 0; m0 < nmod; m0 += group, ++ngroups) {
         const int gn = std::min(group, nmod - m0);
         OzResParams rp;
         std::memset(&rp, 0, sizeof rp);
@@ -854,6 +858,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
             outputs(2, ic.needs_rot == 2 ? 4 : -1);
             HIPCHK(launch_residues(rp, (int)gp, ctx->stream));
         }
+        HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups], ctx->stream));
         for (int i = 0; i < gn; ++i) {
             const int m = m0 + i;
             for (int c = 0; c < nchunk; ++c) {
@@ -869,6 +874,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
                                     (const TileWork*)ctx->oz_work.p, (int)work.size(), md, ctx->stream));
             }
         }
+        HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups + 1], ctx->stream));
     }
     OzFinalParams fp;
     fp.ldacc = ng;
@@ -894,38 +900,161 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         fp.ldc = ldH;
         HIPCHK(launch_crt_final(fp, ctx->stream));
     }
-    // Error estimate.  Rounding an operand element to beta bits leaves an
-    // independent error uniform within +-2^(e_col - beta - 1) in each real
-    // part, so E|dC_ij|^2 = 2^-2beta / 6 (2^(2 eL_i) |R_j|^2 + 2^(2 eR_j) |L_i|^2);
-    // summed over the lower triangle (about half of all i, j) and divided by
-    // the computed |tril(C)|_F.
-    if (do_s) HIPCHK(launch_tril_sumsq(dS, ldS, ng, part, kNblk, ctx->stream));
-    if (do_h) HIPCHK(launch_tril_sumsq(dH, ldH, ng, part + kNblk, kNblk, ctx->stream));
-    std::vector<double> hss(3 * (size_t)gp + 2 * kNblk);
+    // the host vectors above must outlive the async copies
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *ms_igemm = 0.0;
+    for (int g = 0; g < ngroups; ++g) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, ctx->oz_ev[2 * g], ctx->oz_ev[2 * g + 1]));
+        *ms_igemm += t;
+    }
+    return HSGEN_OK;
+}
+
+// S = Z^H Zs and H = Z^H W from the resident FP64 operands, on INT8 MFMA.
+// With nmod_req == 0 the modulus count is chosen for `target`: an FP64 column
+// block of S and H (the middle bn columns, all rows) gives |tril C|_F for the
+// error model before any INT8 work; after the run the model is evaluated with
+// the exact norms and the run repeated with more moduli if it misses the
+// target.  The same block measures the actual error of the result.
+hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod_req,
+                       double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, OzResult* res)
+{
+    const IgemmLaunchCfg ic = igemm_cfg(icfg);
+    const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
+    const int64_t gp = geo.gp;
+    const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
+    if (gp % ic.bm != 0 || gp % ic.bn != 0)
+        return fail(ctx, HSGEN_ERR_ARG, "column padding %lld is not a multiple of the INT8 tile", (long long)gp);
+    if (!(target > 0.0)) target = 1e-12;
+    const int bmin = oz_beta_min(kd);
+    if (nmod_req > 0 && (nmod_req > kMaxModuli || oz_beta(nmod_req) < bmin))
+        return fail(ctx, HSGEN_ERR_ACCURACY, "%d moduli give %d bits per element, depth %lld needs %d..52",
+                    nmod_req, nmod_req > kMaxModuli ? 0 : oz_beta(nmod_req), (long long)kd, bmin);
+    // per-column norm exponents and squared norms of Z, Zs, W; sample statistics
+    const int nb = std::min(ng, geo.t.bn);
+    const int j0 = (ng - nb) / 2;
+    const int kNblk = 1024;
+    hsgen_status st = ensure(ctx, ctx->oz_e, (sizeof(int32_t) + sizeof(double)) * 3 * (size_t)gp +
+                                                 sizeof(double) * (2 * kNblk + 2 * 3 * (size_t)nb), false);
+    if (st) return st;
+    double* ssZ = (double*)ctx->oz_e.p;
+    double* ssZs = ssZ + gp;
+    double* ssW = ssZs + gp;
+    double* part = ssW + gp;                       // 2 x kNblk partial sums of |tril C|^2
+    double* spart = part + 2 * kNblk;              // 2 x 3 nb sample statistics (S, H)
+    int32_t* eZ = (int32_t*)(spart + 2 * 3 * nb);
+    int32_t* eZs = eZ + gp;
+    int32_t* eW = eZs + gp;
+    const double2* Z = (const double2*)ctx->Z.p;
+    const double2* Zs = (const double2*)ctx->Zs.p;
+    const double2* W = (const double2*)ctx->W.p;
+    HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ssZ, ctx->stream));
+    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
+    if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ssW, ctx->stream));
+    // FP64 sample: columns j0 .. j0+nb-1 of S and H, all rows, ld = ng
+    st = ensure(ctx, ctx->oz_smp, sizeof(double2) * 2 * (size_t)ng * nb, false);
+    if (st) return st;
+    double2* smpS = (double2*)ctx->oz_smp.p;
+    double2* smpH = smpS + (size_t)ng * nb;
+    {
+        std::vector<GemmProblem> probs;
+        std::vector<TileWork> work;
+        const int ntr = (ng + geo.t.bm - 1) / geo.t.bm;
+        auto add = [&](const double2* R, double2* C) {
+            GemmProblem p;
+            p.L = Z;
+            p.R = R + (int64_t)j0 * geo.ldz;
+            p.C = C;
+            p.ldl = p.ldr = geo.ldz;
+            p.ldc = ng;
+            p.m = ng;
+            p.n = nb;
+            p.kdepth = (int32_t)geo.kpad;
+            p.lower = 0;
+            for (int bi = 0; bi < ntr; ++bi) work.push_back(TileWork{(int)probs.size(), bi, 0, 0});
+            probs.push_back(p);
+        };
+        if (do_s) add(Zs, smpS);
+        if (do_h) add(W, smpH);
+        st = run_grouped(ctx, geo.cfg, probs, work);
+        if (st) return st;
+    }
+    if (do_s) HIPCHK(launch_block_cmp(nullptr, 0, smpS, ng, ng, nb, j0, 0, spart, ctx->stream));
+    if (do_h) HIPCHK(launch_block_cmp(nullptr, 0, smpH, ng, ng, nb, j0, 0, spart + 3 * nb, ctx->stream));
+    std::vector<double> hss(3 * (size_t)gp + 2 * kNblk + 6 * (size_t)nb);
     std::vector<int32_t> he(3 * (size_t)gp);
     HIPCHK(hipMemcpyAsync(hss.data(), ssZ, sizeof(double) * hss.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(he.data(), eZ, sizeof(int32_t) * he.size(), hipMemcpyDeviceToHost, ctx->stream));
-    // the host vectors above must outlive the async copies
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    auto estimate = [&](int which) {   // 0: S (R = Zs), 1: H (R = W)
-        const double* sl = hss.data();
-        const double* sr = hss.data() + (which == 0 ? 1 : 2) * gp;
-        const int32_t* el = he.data();
-        const int32_t* er = he.data() + (which == 0 ? 1 : 2) * gp;
-        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-        for (int i = 0; i < ng; ++i) {
-            a += std::ldexp(1.0, 2 * (el[i] - pl.beta));
-            b += sr[i];
-            c += std::ldexp(1.0, 2 * (er[i] - pl.beta));
-            d += sl[i];
+    const double* hs_part = hss.data() + 3 * (size_t)gp;
+    const double* hs_spart = hs_part + 2 * kNblk;
+    // |tril C|^2 from the sample: |C|^2 ~ (ng / nb) sum |C_blk|^2, the diagonal
+    // likewise, and |tril C|^2 = (|C|^2 + |diag C|^2) / 2 for Hermitian C
+    double tril2_smp[2] = {0.0, 0.0};
+    for (int w = 0; w < 2; ++w) {
+        double y = 0.0, dg = 0.0;
+        for (int j = 0; j < nb; ++j) {
+            y += hs_spart[3 * (w * nb + j)];
+            dg += hs_spart[3 * (w * nb + j) + 2];
         }
-        double n2 = 0.0;
-        for (int k = 0; k < kNblk; ++k) n2 += hss[3 * (size_t)gp + which * kNblk + k];
-        const double e2 = (a * b + c * d) / 12.0;
-        return n2 > 0.0 ? std::sqrt(e2 / n2) : 0.0;
+        tril2_smp[w] = 0.5 * (y + dg) * (double)ng / (double)nb;
+    }
+    auto model = [&](int which, int beta, double tril2) {   // 0: S (R = Zs), 1: H (R = W)
+        const size_t o = (which == 0 ? 1 : 2) * (size_t)gp;
+        return oz_model(he.data(), hss.data(), he.data() + o, hss.data() + o, ng, beta, tril2);
     };
-    *est_s = do_s ? estimate(0) : 0.0;
-    *est_h = do_h ? estimate(1) : 0.0;
+    auto worst_model = [&](int beta, const double* tril2) {
+        return std::max(do_s ? model(0, beta, tril2[0]) : 0.0, do_h ? model(1, beta, tril2[1]) : 0.0);
+    };
+    int nmod = nmod_req;
+    if (nmod <= 0) {
+        for (int n = 2; n <= kMaxModuli && nmod <= 0; ++n)
+            if (oz_beta(n) >= bmin && worst_model(oz_beta(n), tril2_smp) <= 0.75 * target) nmod = n;
+        if (nmod <= 0)
+            return fail(ctx, HSGEN_ERR_ACCURACY, "no modulus count in [2, %d] reaches rel err %.3g", kMaxModuli,
+                        target);
+    }
+    for (;;) {
+        st = oz_execute(ctx, geo, ng, flags, icfg, nmod, eZ, eZs, eW, dH, ldH, dS, ldS, &res->ms_igemm);
+        if (st) return st;
+        // modelled error with the exact |tril C|
+        if (do_s) HIPCHK(launch_tril_sumsq(dS, ldS, ng, part, kNblk, ctx->stream));
+        if (do_h) HIPCHK(launch_tril_sumsq(dH, ldH, ng, part + kNblk, kNblk, ctx->stream));
+        std::vector<double> hp(2 * kNblk);
+        HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        double tril2[2] = {0.0, 0.0};
+        for (int k = 0; k < kNblk; ++k) {
+            tril2[0] += hp[k];
+            tril2[1] += hp[kNblk + k];
+        }
+        const int beta = oz_beta(nmod);
+        res->nmod = nmod;
+        res->est_s = do_s ? model(0, beta, tril2[0]) : 0.0;
+        res->est_h = do_h ? model(1, beta, tril2[1]) : 0.0;
+        const double worst = std::max(res->est_s, res->est_h);
+        if (nmod_req > 0 || worst <= target || nmod >= kMaxModuli) break;
+        // each modulus adds log2(p) / 2 ~ 3.9 bits to beta
+        const int extra = (int)std::ceil((std::log2(worst / target) + 1.0) / 3.9);
+        nmod = std::min(kMaxModuli, nmod + std::max(extra, 1));
+    }
+    // measured error on the sample block (lower part, rows i >= column)
+    if (do_s) HIPCHK(launch_block_cmp(dS, ldS, smpS, ng, ng, nb, j0, 1, spart, ctx->stream));
+    if (do_h) HIPCHK(launch_block_cmp(dH, ldH, smpH, ng, ng, nb, j0, 1, spart + 3 * nb, ctx->stream));
+    std::vector<double> hsp(6 * (size_t)nb);
+    HIPCHK(hipMemcpyAsync(hsp.data(), spart, sizeof(double) * hsp.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int w = 0; w < 2; ++w) {
+        double y = 0.0, d = 0.0;
+        for (int j = 0; j < nb; ++j) {
+            y += hsp[3 * (w * nb + j)];
+            d += hsp[3 * (w * nb + j) + 1];
+        }
+        const double e = y > 0.0 ? std::sqrt(d / y) : 0.0;
+        if (w == 0) res->smp_s = do_s ? e : 0.0;
+        else res->smp_h = do_h ? e : 0.0;
+    }
     return HSGEN_OK;
 }
 
@@ -971,28 +1100,23 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         add_tmul_problems(probs, work, s, geo, ng, Z, W);
         st = run_grouped(ctx, geo.cfg, probs, work);
         if (st) return st;
+        // rows 2K .. ldz-1 of W are not written by the T application; clear
+        // them so that no earlier, larger system's values remain there (the
+        // split-precision column scaling reads whole columns)
+        const int64_t r0 = 2 * (int64_t)s->kdim;
+        if (geo.ldz > r0)
+            HIPCHK(hipMemset2DAsync(W + r0, sizeof(double2) * geo.ldz, 0, sizeof(double2) * (geo.ldz - r0),
+                                    geo.gp, ctx->stream));
     }
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     probs.clear();
     work.clear();
-    int nmod_used = 0, icfg = 0;
-    double est_h = 0.0, est_s = 0.0;
+    int icfg = 0;
+    OzResult oz;
     if (o.precision == HSGEN_PREC_OZAKI_INT8) {
         icfg = (o.tile > 0 && igemm_cfg(o.tile).bm > 0) ? o.tile : kIgemmCfg;
-        st = run_ozaki(ctx, geo, ng, flags, icfg, o.slices, o.target_rel_err, dH, ldH, dS, ldS, &nmod_used,
-                       &est_h, &est_s);
+        st = run_ozaki(ctx, geo, ng, flags, icfg, o.slices, o.target_rel_err, dH, ldH, dS, ldS, &oz);
         if (st) return st;
-        // automatic modulus count: check the estimate and redo with enough extra
-        // moduli (each adds ~4 bits to beta) when the target is missed
-        const double target = o.target_rel_err > 0.0 ? o.target_rel_err : 1e-12;
-        const double worst = std::max(est_h, est_s);
-        if (o.slices == 0 && worst > target && nmod_used < kMaxModuli) {
-            const int extra = (int)std::ceil((std::log2(worst / target) + 1.0) / 3.9);
-            const int n2 = std::min(kMaxModuli, nmod_used + std::max(extra, 1));
-            st = run_ozaki(ctx, geo, ng, flags, icfg, n2, target, dH, ldH, dS, ldS, &nmod_used, &est_h,
-                           &est_s);
-            if (st) return st;
-        }
     } else {
         add_hs_problems(probs, work, geo, ng, Z, Zs, W, dH, dS, 0, flags);
         xcd_permute(work);
@@ -1029,9 +1153,12 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         rep->ng = ng;
         rep->kdim = s->kdim;
         rep->flops_alg = 16.0 * (double)ng * ng * s->kdim + 32.0 * (double)ng * (double)s->sum_nlm2;
-        rep->slices_used = nmod_used;
-        rep->err_est_h = est_h;
-        rep->err_est_s = est_s;
+        rep->slices_used = oz.nmod;
+        rep->err_est_h = oz.est_h;
+        rep->err_est_s = oz.est_s;
+        rep->ms_igemm = oz.ms_igemm;
+        rep->err_smp_h = oz.smp_h;
+        rep->err_smp_s = oz.smp_s;
         rep->tile_used = o.precision == HSGEN_PREC_OZAKI_INT8 ? icfg : geo.cfg;
     }
     return HSGEN_OK;

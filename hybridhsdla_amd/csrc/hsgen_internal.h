@@ -157,6 +157,8 @@ struct OzFinalParams {
 hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, double* ss,
                          hipStream_t st);
 hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st);
+hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
+                            int j0, int lower, double* part, hipStream_t st);
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);
 IgemmLaunchCfg igemm_cfg(int id);
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,

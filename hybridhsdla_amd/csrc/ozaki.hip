@@ -3,8 +3,9 @@
 // Chinese-remainder recombination in FP64).
 //
 // For C = L^H R (complex, depth kd):
-//   1. per column j of each FP64 operand X: e_j with max_k |X_kj| < 2^e_j
-//      (re and im), X'_kj = rint(X_kj 2^(beta - e_j)), |X'| <= 2^beta;
+//   1. per column j of each FP64 operand X: e_j with ||X_:j||_2 < 2^e_j,
+//      X'_kj = rint(X_kj 2^(beta - e_j)), so |X'| < 2^beta and, rounding
+//      included, ||X'_:j||_2 < 2^beta (1 + 2^-20) for beta >= 19 + log2(2 kd) / 2;
 //   2. for each modulus p_m (<= 255): the symmetric residues X' mod p_m fit
 //      INT8; L is stored twice, as (Lr, Li) and as (-Li, Lr) pairs, so that
 //        Re C' = sum (Lr Rr + Li Ri),   Im C' = sum (-Li Rr + Lr Ri)
@@ -17,8 +18,9 @@
 //      sum is C'/P modulo 1, exact up to the double-double rounding;
 //   5. the final pass takes the centred fractional part, multiplies by P and
 //      rescales by 2^(e_i + e_j - 2 beta).
-// P > 2 max|C'| = 2 * 2kd * 2^(2 beta) is chosen by the host, which sets beta
-// from the number of moduli.
+// By Cauchy-Schwarz |C'_ij| <= ||L'_:i|| ||R'_:j|| < 2^(2 beta) (1 + 2^-19), so
+// P > 2^(2 beta + 1.01) suffices whatever the depth; the host sets beta from
+// the number of moduli (beta <= 52 keeps every X' an exact double).
 #include "hsgen_internal.h"
 
 namespace hsgen {
@@ -29,33 +31,31 @@ typedef __attribute__((address_space(1))) const i4v* gi4ptr;
 // ---------------------------------------------------------------------------
 // 1. column exponents
 // ---------------------------------------------------------------------------
-// also the column's squared 2-norm (for the error estimate)
+// e_j = exponent of the column's 2-norm (||X_:j|| < 2^e_j), and the squared
+// norm itself (for the error estimate)
 __global__ __launch_bounds__(256) void k_colexp(const double2* __restrict__ X, int64_t ldx, int rows,
                                                 int ncols, int32_t* __restrict__ e, double* __restrict__ ss)
 {
-    __shared__ double red[256], red2[256];
+    __shared__ double red2[256];
     const int col = blockIdx.x;
     if (col >= ncols) return;
     const double2* c = X + (int64_t)col * ldx;
-    double m = 0.0, q = 0.0;
+    double q = 0.0;
     for (int r = threadIdx.x; r < rows; r += blockDim.x) {
         double2 v = c[r];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
         q = fma(v.x, v.x, fma(v.y, v.y, q));
     }
-    red[threadIdx.x] = m;
     red2[threadIdx.x] = q;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-            red2[threadIdx.x] += red2[threadIdx.x + s];
-        }
+        if (threadIdx.x < s) red2[threadIdx.x] += red2[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         int ex = 0;
-        if (red[0] > 0.0) (void)frexp(red[0], &ex);   // red[0] < 2^ex
+        // frexp gives 2^(ex-1) <= norm < 2^ex; the summation and the square root
+        // are accurate to a few ulps, far inside the 2^-19 slack of the bound
+        if (red2[0] > 0.0) (void)frexp(sqrt(red2[0]), &ex);
         e[col] = ex;
         ss[col] = red2[0];
     }
@@ -92,6 +92,59 @@ __global__ __launch_bounds__(256) void k_tril_sumsq(const double2* __restrict__ 
 hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(k_tril_sumsq, dim3(nblk), dim3(256), 0, st, C, ldc, n, part);
+    return hipGetLastError();
+}
+
+// Statistics of a column block Y (m x n, ld ldy) whose column j is column
+// j0 + j of a square matrix, one block per column:
+//   part[3 j]     = sum |Y_ij|^2,
+//   part[3 j + 1] = sum |X_i,j0+j - Y_ij|^2   (0 when X is null),
+//   part[3 j + 2] = |Y_j0+j,j|^2              (the diagonal element),
+// over all rows i < m, or over i >= j0 + j when lower != 0.
+__global__ __launch_bounds__(256) void k_block_cmp(const double2* __restrict__ X, int64_t ldx,
+                                                   const double2* __restrict__ Y, int64_t ldy, int m,
+                                                   int j0, int lower, double* __restrict__ part)
+{
+    __shared__ double ry[256], rd[256];
+    const int j = blockIdx.x;
+    const int i0 = lower ? j0 + j : 0;
+    double qy = 0.0, qd = 0.0;
+    for (int i = i0 + threadIdx.x; i < m; i += blockDim.x) {
+        const double2 y = Y[(int64_t)j * ldy + i];
+        qy = fma(y.x, y.x, fma(y.y, y.y, qy));
+        if (X) {
+            const double2 x = X[(int64_t)(j0 + j) * ldx + i];
+            const double dx = x.x - y.x, dy = x.y - y.y;
+            qd = fma(dx, dx, fma(dy, dy, qd));
+        }
+    }
+    ry[threadIdx.x] = qy;
+    rd[threadIdx.x] = qd;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ry[threadIdx.x] += ry[threadIdx.x + s];
+            rd[threadIdx.x] += rd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * j] = ry[0];
+        part[3 * j + 1] = rd[0];
+        double dg = 0.0;
+        if (j0 + j < m) {
+            const double2 y = Y[(int64_t)j * ldy + j0 + j];
+            dg = y.x * y.x + y.y * y.y;
+        }
+        part[3 * j + 2] = dg;
+    }
+}
+
+hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
+                            int j0, int lower, double* part, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_block_cmp, dim3(n), dim3(256), 0, st, X, ldx, Y, ldy, m, j0, lower, part);
     return hipGetLastError();
 }
 

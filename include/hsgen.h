@@ -89,8 +89,9 @@ typedef struct {
 typedef struct {
     int32_t precision;         /* hsgen_precision */
     int32_t slices;            /* split precision: INT8 residue slices per operand, i.e. the
-                                  number of moduli (0 = chosen for target_rel_err and checked
-                                  afterwards against the error estimate, with a retry) */
+                                  number of moduli (0 = chosen for target_rel_err from an error
+                                  model and an FP64 sample block of H and S, then checked
+                                  against the model with the exact norms, with a retry) */
     double target_rel_err;     /* split precision: target relative Frobenius error (0 = 1e-12) */
     int32_t flags;             /* HSGEN_FILL_UPPER | HSGEN_SKIP_H | HSGEN_SKIP_S */
     int32_t tile;              /* contraction tile config id (0 = autotuned default) */
@@ -110,6 +111,11 @@ typedef struct {
     int32_t tile_used;         /* contraction tile config id actually used */
     double err_est_h;          /* split precision: estimated relative Frobenius error of H */
     double err_est_s;          /* ... and of S (0 in FP64) */
+    double ms_igemm;           /* split precision: GPU time of the INT8 GEMM launches alone
+                                  (last modulus set when an automatic retry ran) */
+    double err_smp_h;          /* split precision: measured relative Frobenius error of H on a
+                                  block of bn columns (lower part) recomputed in FP64 */
+    double err_smp_s;          /* ... and of S */
 } hsgen_report;
 
 /* Context: bind GPU `device` (HIP ordinal), create its stream.
@@ -177,6 +183,44 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G,
                                const double* sqrt_u_norms,
                                const hsgen_opts* opts, double* H, double* S,
                                int64_t ld, hsgen_report* report);
+
+/* ---------------------------------------------------------------------------
+ * System set-up on the host (no GPU needed): what turns a Fleur dump into the
+ * hsgen_system above.
+ * ------------------------------------------------------------------------ */
+
+/* Scalar-relativistic radial function of angular momentum l at energy E and
+ * its energy derivative for the spherical potential vr = r V(r) (Hartree) on
+ * the logarithmic mesh r_i = rmt exp(dx (i + 1 - n)), i = 0..n-1 (n >= 7).
+ * match[5] receives {u(R), u'(R), u_dot(R), u_dot'(R), <u_dot|u_dot>} at
+ * R = rmt (the rad_match order, types.h:35-43).  f and g may be NULL, else
+ * n x 2 column-major: the normalised large and small components of r u and of
+ * r u_dot.  Replaces radial_solve (special_funcs.cpp:21-36) -> wrap_radfun
+ * (libflerp.f90:50-93) -> Fleur's radfun/radsra/radsrd/intgr0
+ * (radfun.f:3-59, radsra.f:10-120, radsrd.F:30-160, intgr.F:58-116). */
+hsgen_status hsgen_radial_solve(int32_t l, double E, double dx, double rmt, int32_t n,
+                                const double* vr, double* match, double* f, double* g);
+
+/* Integer quadrature and multistep weights the radial solver derives from the
+ * Lagrange interpolation polynomials: which = 0: 5 x 7 first-interval weights
+ * over 60480 (row j integrates [t_j, t_j+1] from nodes 0..6); 1: 7 closed
+ * Newton-Cotes weights over 140; 2: 6 Adams-Bashforth weights over 1440;
+ * 3: 6 Adams-Moulton weights over 1440.  (The tables of intgr.F:39-50 and
+ * radsra.f:92-104; exported for tests.) */
+hsgen_status hsgen_quadrature_weights(int32_t which, double* w);
+
+/* Final T-matrices of one atom from the dumped blocks of its type.
+ * t4: 4 x n x n complex, n = (lmax+1)^2, column-major slices t_uu, t_dd,
+ * t_ud, t_du of which only the lower triangles are read (the /tmat/<type>
+ * dataset layout); E[lmax+1] the energy parameters, ddn[lmax+1] the
+ * <u_dot|u_dot> of the atom.  Writes full n x n complex T_aa, T_ab, T_bb
+ * (T_aa, T_bb Hermitian; T_ba = T_ab^H).  Replaces the TMat constructor,
+ * TMat::zero_nonsph and AtomData::augment_tmats (types_helper.cpp:41-88,
+ * AtomData.cpp:95-145).  HSGEN_ERR_ARG if the diagonals of t_ud and t_du
+ * differ (they are one matrix). */
+hsgen_status hsgen_tmat_assemble(int32_t lmax, int32_t lnonsph, const double* t4,
+                                 const double* E, const double* ddn, double* t_aa,
+                                 double* t_ab, double* t_bb);
 
 #ifdef __cplusplus
 }
