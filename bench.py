@@ -33,6 +33,7 @@ WORKLOAD = {
     0: "cfg0: 4 atoms/2 types, lmax 8 (L=81), a=10 bohr, k=(0.1,0.2,0.3)",
     1: "cfg1: 32 atoms/4 types, lmax 10 (L=121), a=20 bohr, k=(0.1,0.2,0.3)",
     2: "cfg2: 256 atoms/8 types, lmax 12 (L=169), a=40 bohr, k=(0.1,0.2,0.3)",
+    4: "cfg4: 48 atoms/3 types, lmax 8/10/12 by type, a=24 bohr, 4x4x4 Monkhorst-Pack k-points",
 }
 
 
@@ -60,8 +61,9 @@ def cpu_baseline(cfg: int, budget_s: float):
     if not po.blas_ready(threads):
         return None
     system, kps = make_config(cfg)
-    full_ng = kps[0].ng
-    k = kps[0].k
+    kp0 = max(kps, key=lambda kp: kp.ng)
+    full_ng = kp0.ng
+    k = kp0.k
     b33 = system.bmat33()
 
     def run(ng):
@@ -189,6 +191,100 @@ def split_run(args, system, G, H, S, step, fp64_ms, F):
     }
 
 
+def bench_batched(args) -> int:
+    """64 k-points of uneven N_G through hsgen_create_hs_batched (grouped
+    launches, graph captured on the warm-up call and replayed)."""
+    import numpy as np
+    import torch
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+
+    t0 = time.perf_counter()
+    system, kps = make_config(4)
+    ngs = [kp.ng for kp in kps]
+    log(f"[bench] workload cfg4: natoms={system.natoms} K={system.kdim} nk={len(kps)} "
+        f"N_G {min(ngs)}..{max(ngs)} (setup {time.perf_counter() - t0:.1f} s)")
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    sysh = ctx.upload(system)
+    mats = [(torch.zeros((n, n), dtype=torch.complex128, device=dev),
+             torch.zeros((n, n), dtype=torch.complex128, device=dev)) for n in ngs]
+    outs = [(a.data_ptr(), b.data_ptr(), n) for (a, b), n in zip(mats, ngs)]
+    prec = HSGEN_PREC_FP64 if args.precision == "fp64" else HSGEN_PREC_OZAKI_INT8
+
+    def step():
+        return ctx.create_HS2_batched(sysh, kps, precision=prec, target_rel_err=args.split_target,
+                                      tile=args.tile, outputs=outs)[1]
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"[bench] warmup {i}: {r[0]['ms_total']:.1f} ms")
+    torch.cuda.synchronize()
+    reps = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        reps.append(step())
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t_start) * 1e3 / args.steps
+    F = float(sum(system.flops_alg(n) for n in ngs))
+    f_contract = float(sum(16.0 * n * n * system.kdim for n in ngs))
+    ms_gpu = float(np.mean([r[0]["ms_total"] for r in reps]))
+    tflops = F / (ms_step * 1e-3) / 1e12
+    peak = FP64_PEAK if prec == HSGEN_PREC_FP64 else INT8_PEAK
+    check = None
+    if not args.no_check:
+        try:
+            i = int(np.argmax(ngs))
+            kp = kps[i]
+            check = sampled_parity(system, kp, mats[i][0], mats[i][1], kp.ng)
+            log(f"[bench] sampled parity (k {i}, N_G {kp.ng}): {check}")
+        except Exception as e:
+            check = {"error": str(e)}
+    # the same k-points one call each (no grouping, no graph), for comparison
+    loop = None
+    if not args.no_split:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for kp, (a, b) in zip(kps, mats):
+            ctx.create_hs_device(sysh, kp, a.data_ptr(), b.data_ptr(), kp.ng, precision=prec,
+                                 target_rel_err=args.split_target, tile=args.tile)
+        torch.cuda.synchronize()
+        loop_ms = (time.perf_counter() - t1) * 1e3
+        loop = {"ms_per_step": round(loop_ms, 3), "batched_speedup": round(loop_ms / ms_step, 3)}
+        log(f"[bench] one call per k-point: {loop}")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(4, args.cpu_budget)
+        except Exception as e:
+            cpu = {"error": str(e)}
+    out = {
+        "metric": METRIC, "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "sec_per_hs": round(ms_step * 1e-3 / len(kps), 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == HSGEN_PREC_FP64 else "int8-slices/f64",
+        "data": "synthetic (seeded system generator; random-init T-matrices and radial values)",
+        "config": {"workload": WORKLOAD[4] + f", N_G {min(ngs)}..{max(ngs)}, K={system.kdim}",
+                   "natoms": system.natoms, "n_k": len(kps), "n_g_total": int(sum(ngs)),
+                   "k_dim": system.kdim, "precision": args.precision},
+        "flops_per_step": F,
+        "roofline": {"bound": "mfma", "kernel": "batched graph: k_abgen per k, grouped T application, "
+                     "persistent grouped S/H contraction",
+                     "achieved": round(f_contract / (ms_gpu * 1e-3) / 1e12, 3), "peak": peak,
+                     "unit": "TFLOP/s" if prec == HSGEN_PREC_FP64 else "TOP/s",
+                     "frac": round(f_contract / (ms_gpu * 1e-3) / 1e12 / peak, 4), "traffic": None,
+                     "note": "contraction flops over the whole graph's GPU time"},
+        "parity_sample": check,
+        "per_k_calls": loop,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    sysh.close()
+    ctx.close()
+    return 0
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM traffic per launch from the committed rocprofv3 --pmc summary."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_traffic.json"
@@ -205,7 +301,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 4],
+                    help="workload; 4 = 64 k-points through the batched entry point")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "int8"])
     ap.add_argument("--slices", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
@@ -228,6 +325,9 @@ def main(argv=None) -> int:
     import torch
     from hybridhsdla_amd import make_config
     from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+
+    if args.config == 4:
+        return bench_batched(args)
 
     t0 = time.perf_counter()
     system, kps = make_config(args.config)

@@ -96,6 +96,8 @@ SIGNATURES = [
      [_p, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, C.POINTER(_p), C.POINTER(_p),
       C.POINTER(_p), _p, C.c_int32, _p, C.POINTER(HsgenOpts), _p, _p, C.c_int64,
       C.POINTER(HsgenReport)]),
+    ("hsgen_autotune", C.c_int,
+     [_p, _p, C.POINTER(HsgenKpoint), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     ("hsgen_radial_solve", C.c_int,
      [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, _p, _p, _p, _p]),
     ("hsgen_quadrature_weights", C.c_int, [C.c_int32, _p]),

@@ -179,6 +179,16 @@ class Context:
                                             s_ptr, int(ld), C.byref(rep)), self.handle)
         return rep.as_dict()
 
+    def autotune(self, sysh: SystemHandle, kp: KPoint, precision=HSGEN_PREC_FP64):
+        """Time the tile configurations on kp, remember the fastest for this
+        problem size; returns (config id, contraction ms)."""
+        ks, g = _kpoint(kp)
+        best = C.c_int32()
+        ms = C.c_double()
+        _lib.check(self.lib.hsgen_autotune(self.handle, sysh.handle, C.byref(ks), int(precision), C.byref(best),
+                                           C.byref(ms)), self.handle)
+        return best.value, ms.value
+
     def create_HS2_batched(self, sysh: SystemHandle, kps: Sequence[KPoint], precision=HSGEN_PREC_FP64,
                            slices=0, target_rel_err=0.0, flags=0, tile=0, outputs=None):
         """H, S for many k-points; outputs = list of (H_ptr, S_ptr, ld) device

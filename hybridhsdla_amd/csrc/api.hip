@@ -10,6 +10,7 @@
 // ldz = roundup(2K, BK) + BK, columns padded to the tile size; every padding
 // element is zero so the kernels need no bounds checks along k.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -43,9 +44,22 @@ struct hsgen_ctx {
     // split precision
     DevBuf oz_e, oz_res, oz_dd, oz_probs, oz_work, oz_smp;
     hipEvent_t oz_ev[2 * kMaxModuli] = {};   // INT8 GEMM start/stop per residue group
+    int n_cu = 0;                             // compute units (persistent grids)
+    // batched k-points: one captured graph per batch, replayed while the
+    // call signature is unchanged
+    struct BatchGraph {
+        std::vector<int64_t> sig;
+        hipGraphExec_t exec = nullptr;
+        DevBuf g, probs, work;
+    };
+    std::vector<BatchGraph> bgraphs;
+    DevBuf btmp;                              // device H, S of host-output k-points
+    // autotuned tile configurations: (precision, size buckets) -> config id
+    std::map<std::vector<int>, int> tuned;
 };
 
 struct hsgen_sys {
+    uint64_t serial = 0;               // unique per created system
     int natoms = 0, max_lmax = 0, nl = 0, L = 0, kdim = 0;
     double omega = 0.0, vol_prefac = 0.0;
     double bmat[9];
@@ -353,6 +367,7 @@ hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HSGEN_ERR_NODEV;
     ctx = new hsgen_ctx();
     ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
@@ -368,8 +383,15 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
                       &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp, &ctx->oz_e, &ctx->oz_res,
-                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp})
+                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp, &ctx->btmp})
         release(*b);
+    for (auto& bg : ctx->bgraphs) {
+        if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
+        release(bg.g);
+        release(bg.probs);
+        release(bg.work);
+    }
+    ctx->bgraphs.clear();
     return HSGEN_OK;
 }
 
@@ -401,7 +423,9 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
         return fail(ctx, HSGEN_ERR_ARG, "null array in system description");
     if (!(d->omega > 0.0)) return fail(ctx, HSGEN_ERR_ARG, "omega must be > 0");
     HIPCHK(hipSetDevice(ctx->device));
+    static std::atomic<uint64_t> next_serial{0};
     hsgen_sys* s = new hsgen_sys();
+    s->serial = ++next_serial;
     s->natoms = d->natoms;
     s->max_lmax = d->max_lmax;
     s->nl = d->max_lmax + 1;
@@ -912,11 +936,11 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
 }
 
 // S = Z^H Zs and H = Z^H W from the resident FP64 operands, on INT8 MFMA.
-// With nmod_req == 0 the modulus count is chosen for `target`: an FP64 column
-// block of S and H (the middle bn columns, all rows) gives |tril C|_F for the
-// error model before any INT8 work; after the run the model is evaluated with
-// the exact norms and the run repeated with more moduli if it misses the
-// target.  The same block measures the actual error of the result.
+// With nmod_req == 0 the modulus count is chosen for `target`: bn evenly
+// spaced columns of S and H computed in FP64 (all rows) give |tril C|_F for
+// the error model before any INT8 work; after the run the model is evaluated
+// with the exact norms and the run repeated with more moduli if it misses the
+// target.  The same sample measures the actual error of the result.
 hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod_req,
                        double target, double2* dH, int64_t ldH, double2* dS, int64_t ldS, OzResult* res)
 {
@@ -933,10 +957,10 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
                     nmod_req, nmod_req > kMaxModuli ? 0 : oz_beta(nmod_req), (long long)kd, bmin);
     // per-column norm exponents and squared norms of Z, Zs, W; sample statistics
     const int nb = std::min(ng, geo.t.bn);
-    const int j0 = (ng - nb) / 2;
     const int kNblk = 1024;
     hsgen_status st = ensure(ctx, ctx->oz_e, (sizeof(int32_t) + sizeof(double)) * 3 * (size_t)gp +
-                                                 sizeof(double) * (2 * kNblk + 2 * 3 * (size_t)nb), false);
+                                                 sizeof(double) * (2 * kNblk + 2 * 3 * (size_t)nb) +
+                                                 sizeof(int32_t) * (size_t)nb, false);
     if (st) return st;
     double* ssZ = (double*)ctx->oz_e.p;
     double* ssZs = ssZ + gp;
@@ -946,17 +970,29 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     int32_t* eZ = (int32_t*)(spart + 2 * 3 * nb);
     int32_t* eZs = eZ + gp;
     int32_t* eW = eZs + gp;
+    int32_t* dcols = eW + gp;                      // the sampled columns
+    std::vector<int32_t> cols(nb);
+    for (int j = 0; j < nb; ++j) cols[j] = (int32_t)(((2 * (int64_t)j + 1) * ng) / (2 * (int64_t)nb));
+    HIPCHK(hipMemcpyAsync(dcols, cols.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, ctx->stream));
     const double2* Z = (const double2*)ctx->Z.p;
     const double2* Zs = (const double2*)ctx->Zs.p;
     const double2* W = (const double2*)ctx->W.p;
     HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ssZ, ctx->stream));
     if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
     if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ssW, ctx->stream));
-    // FP64 sample: columns j0 .. j0+nb-1 of S and H, all rows, ld = ng
-    st = ensure(ctx, ctx->oz_smp, sizeof(double2) * 2 * (size_t)ng * nb, false);
+    // FP64 sample: columns cols[] of S and H, all rows, ld = ng (the right
+    // operand's sampled columns are gathered first: the GEMM reads contiguous
+    // columns)
+    // (the GEMM reads whole tiles of bn columns: the gathered arrays get bn)
+    const size_t nbp = (size_t)roundup(nb, geo.t.bn);
+    st = ensure(ctx, ctx->oz_smp, sizeof(double2) * 2 * ((size_t)ng * nb + (size_t)geo.ldz * nbp), false);
     if (st) return st;
     double2* smpS = (double2*)ctx->oz_smp.p;
     double2* smpH = smpS + (size_t)ng * nb;
+    double2* gathS = smpH + (size_t)ng * nb;
+    double2* gathH = gathS + (size_t)geo.ldz * nbp;
+    if (do_s) HIPCHK(launch_gather_cols(Zs, geo.ldz, dcols, (int)geo.ldz, nb, gathS, ctx->stream));
+    if (do_h) HIPCHK(launch_gather_cols(W, geo.ldz, dcols, (int)geo.ldz, nb, gathH, ctx->stream));
     {
         std::vector<GemmProblem> probs;
         std::vector<TileWork> work;
@@ -964,7 +1000,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         auto add = [&](const double2* R, double2* C) {
             GemmProblem p;
             p.L = Z;
-            p.R = R + (int64_t)j0 * geo.ldz;
+            p.R = R;
             p.C = C;
             p.ldl = p.ldr = geo.ldz;
             p.ldc = ng;
@@ -975,13 +1011,13 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
             for (int bi = 0; bi < ntr; ++bi) work.push_back(TileWork{(int)probs.size(), bi, 0, 0});
             probs.push_back(p);
         };
-        if (do_s) add(Zs, smpS);
-        if (do_h) add(W, smpH);
+        if (do_s) add(gathS, smpS);
+        if (do_h) add(gathH, smpH);
         st = run_grouped(ctx, geo.cfg, probs, work);
         if (st) return st;
     }
-    if (do_s) HIPCHK(launch_block_cmp(nullptr, 0, smpS, ng, ng, nb, j0, 0, spart, ctx->stream));
-    if (do_h) HIPCHK(launch_block_cmp(nullptr, 0, smpH, ng, ng, nb, j0, 0, spart + 3 * nb, ctx->stream));
+    if (do_s) HIPCHK(launch_block_cmp(nullptr, 0, smpS, ng, ng, nb, dcols, 0, spart, ctx->stream));
+    if (do_h) HIPCHK(launch_block_cmp(nullptr, 0, smpH, ng, ng, nb, dcols, 0, spart + 3 * nb, ctx->stream));
     std::vector<double> hss(3 * (size_t)gp + 2 * kNblk + 6 * (size_t)nb);
     std::vector<int32_t> he(3 * (size_t)gp);
     HIPCHK(hipMemcpyAsync(hss.data(), ssZ, sizeof(double) * hss.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -989,8 +1025,9 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const double* hs_part = hss.data() + 3 * (size_t)gp;
     const double* hs_spart = hs_part + 2 * kNblk;
-    // |tril C|^2 from the sample: |C|^2 ~ (ng / nb) sum |C_blk|^2, the diagonal
-    // likewise, and |tril C|^2 = (|C|^2 + |diag C|^2) / 2 for Hermitian C
+    // |tril C|^2 from the sample: |C|^2 ~ (ng / nb) sum |C_:,cols|^2, the
+    // diagonal likewise, and |tril C|^2 = (|C|^2 + |diag C|^2) / 2 for
+    // Hermitian C
     double tril2_smp[2] = {0.0, 0.0};
     for (int w = 0; w < 2; ++w) {
         double y = 0.0, dg = 0.0;
@@ -1040,8 +1077,8 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         nmod = std::min(kMaxModuli, nmod + std::max(extra, 1));
     }
     // measured error on the sample block (lower part, rows i >= column)
-    if (do_s) HIPCHK(launch_block_cmp(dS, ldS, smpS, ng, ng, nb, j0, 1, spart, ctx->stream));
-    if (do_h) HIPCHK(launch_block_cmp(dH, ldH, smpH, ng, ng, nb, j0, 1, spart + 3 * nb, ctx->stream));
+    if (do_s) HIPCHK(launch_block_cmp(dS, ldS, smpS, ng, ng, nb, dcols, 1, spart, ctx->stream));
+    if (do_h) HIPCHK(launch_block_cmp(dH, ldH, smpH, ng, ng, nb, dcols, 1, spart + 3 * nb, ctx->stream));
     std::vector<double> hsp(6 * (size_t)nb);
     HIPCHK(hipMemcpyAsync(hsp.data(), spart, sizeof(double) * hsp.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1056,6 +1093,22 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         else res->smp_h = do_h ? e : 0.0;
     }
     return HSGEN_OK;
+}
+
+// Size buckets of the autotuning table: half-octaves of G and K.
+std::vector<int> tune_key(int precision, int ng, int kdim)
+{
+    auto b = [](int x) { return (int)std::floor(2.0 * std::log2((double)std::max(x, 1))); };
+    return {precision, b(ng), b(kdim)};
+}
+
+// Tile configuration for a call: the caller's, else the autotuned one for
+// this size, else 0 (the built-in default).
+int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim)
+{
+    if (o.tile > 0) return o.tile;
+    auto it = ctx->tuned.find(tune_key(o.precision, ng, kdim));
+    return it == ctx->tuned.end() ? 0 : it->second;
 }
 
 // Common tail of create_hs / contract_hs: Z, Zs resident -> tmul -> S, H.
@@ -1103,10 +1156,7 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         // rows 2K .. ldz-1 of W are not written by the T application; clear
         // them so that no earlier, larger system's values remain there (the
         // split-precision column scaling reads whole columns)
-        const int64_t r0 = 2 * (int64_t)s->kdim;
-        if (geo.ldz > r0)
-            HIPCHK(hipMemset2DAsync(W + r0, sizeof(double2) * geo.ldz, 0, sizeof(double2) * (geo.ldz - r0),
-                                    geo.gp, ctx->stream));
+        HIPCHK(launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)geo.gp, ctx->stream));
     }
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     probs.clear();
@@ -1160,6 +1210,267 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         rep->err_smp_h = oz.smp_h;
         rep->err_smp_s = oz.smp_s;
         rep->tile_used = o.precision == HSGEN_PREC_OZAKI_INT8 ? icfg : geo.cfg;
+    }
+    return HSGEN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched k-points, FP64: per batch of k-points that fits in memory, A/B
+// generation per k into one column-concatenated Z/Zs, ONE grouped launch for
+// the T application of all their columns and ONE persistent grouped launch
+// for the S and H tiles of all of them (largest k first).  The launch
+// sequence is captured in a hipGraph and replayed while the call signature
+// (system, k-points and their G lists, outputs, workspace) is unchanged, as
+// in repeated SCF iterations over one k-set.
+// ---------------------------------------------------------------------------
+uint64_t hash_words(const void* data, size_t bytes, uint64_t h)
+{
+    const unsigned char* c = (const unsigned char*)data;
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;   // FNV-1a
+    return h;
+}
+
+hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoint* kps, const hsgen_opts& o,
+                          double* const* H, double* const* S, const int64_t* ld, hsgen_report* rep)
+{
+    const int flags = o.flags;
+    const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
+    int max_ng = 1;
+    for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
+    const Geometry geo = geometry(o.tile, s->kdim, max_ng);
+    const int64_t tm = std::max(geo.t.bm, geo.t.bn);
+    hsgen_status st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
+    if (st) return st;
+    // largest k first
+    std::vector<int> order(nk);
+    for (int i = 0; i < nk; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return kps[a].ng > kps[b].ng; });
+    std::vector<bool> hdev(nk), sdev(nk);
+    for (int i = 0; i < nk; ++i) {
+        hdev[i] = !do_h || is_device_ptr(H[i]);
+        sdev[i] = !do_s || is_device_ptr(S[i]);
+    }
+    // batches: operands (Z, Zs, W) plus device copies of host outputs within
+    // ~70% of the memory the call may use
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    size_t budget = (size_t)(0.7 * (double)(fr + ctx->Z.bytes + ctx->Zs.bytes + ctx->W.bytes + ctx->btmp.bytes));
+    if (const char* e = std::getenv("HSGEN_BATCH_BYTES"))   // cap (tuning and tests)
+        if (std::strtoull(e, nullptr, 10) > 0) budget = std::min<size_t>(budget, std::strtoull(e, nullptr, 10));
+    std::vector<std::vector<int>> batches;
+    {
+        size_t used = 0;
+        std::vector<int> cur;
+        for (int i : order) {
+            const size_t gp = (size_t)roundup(std::max(kps[i].ng, 1), tm);
+            const size_t need = 3 * sizeof(double2) * (size_t)geo.ldz * gp +
+                                sizeof(double2) * (size_t)kps[i].ng * kps[i].ng * ((!hdev[i]) + (!sdev[i]));
+            if (!cur.empty() && used + need > budget) {
+                batches.push_back(cur);
+                cur.clear();
+                used = 0;
+            }
+            cur.push_back(i);
+            used += need;
+        }
+        if (!cur.empty()) batches.push_back(cur);
+    }
+    if (ctx->bgraphs.size() < batches.size()) ctx->bgraphs.resize(batches.size());
+    const int pgrid = ctx->n_cu > 0 ? ctx->n_cu : 256;   // one 8-wave workgroup per CU (LDS-limited)
+    double ms_all = 0.0;
+    for (size_t b = 0; b < batches.size(); ++b) {
+        const std::vector<int>& ks = batches[b];
+        std::vector<int64_t> col0(ks.size()), tmp0(ks.size());
+        int64_t ncols = 0, ntmp = 0;
+        for (size_t q = 0; q < ks.size(); ++q) {
+            const int i = ks[q];
+            col0[q] = ncols;
+            ncols += roundup(std::max(kps[i].ng, 1), tm);
+            tmp0[q] = ntmp;
+            ntmp += (int64_t)kps[i].ng * kps[i].ng * ((!hdev[i]) + (!sdev[i]));
+        }
+        const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * ncols;
+        st = ensure(ctx, ctx->Z, zbytes, false);
+        if (st) return st;
+        st = ensure(ctx, ctx->Zs, zbytes, false);
+        if (st) return st;
+        if (do_h) {
+            st = ensure(ctx, ctx->W, zbytes, true);
+            if (st) return st;
+        }
+        if (ntmp > 0) {
+            st = ensure(ctx, ctx->btmp, sizeof(double2) * (size_t)ntmp, false);
+            if (st) return st;
+        }
+        double2* Z = (double2*)ctx->Z.p;
+        double2* Zs = (double2*)ctx->Zs.p;
+        double2* W = (double2*)ctx->W.p;
+        // device outputs of every k of the batch (the caller's, or scratch)
+        std::vector<double2*> dH(ks.size(), nullptr), dS(ks.size(), nullptr);
+        std::vector<int64_t> ldH(ks.size()), ldS(ks.size());
+        for (size_t q = 0; q < ks.size(); ++q) {
+            const int i = ks[q];
+            double2* t0 = (double2*)ctx->btmp.p + tmp0[q];
+            const bool th = do_h && !hdev[i];
+            const bool ts = do_s && !sdev[i];
+            dH[q] = do_h ? (th ? t0 : (double2*)H[i]) : nullptr;
+            dS[q] = do_s ? (ts ? t0 + (th ? (int64_t)kps[i].ng * kps[i].ng : 0) : (double2*)S[i]) : nullptr;
+            ldH[q] = th ? kps[i].ng : ld[i];
+            ldS[q] = ts ? kps[i].ng : ld[i];
+        }
+        // call signature
+        std::vector<int64_t> sig{(int64_t)s->serial, (int64_t)flags, geo.cfg, (int64_t)(intptr_t)Z,
+                                 (int64_t)(intptr_t)Zs, (int64_t)(intptr_t)W, (int64_t)(intptr_t)s->d_M,
+                                 (int64_t)(intptr_t)ctx->btmp.p, (int64_t)ks.size()};
+        for (size_t q = 0; q < ks.size(); ++q) {
+            const int i = ks[q];
+            int64_t kb[3];
+            std::memcpy(kb, kps[i].k, sizeof kb);
+            sig.insert(sig.end(), {(int64_t)i, (int64_t)kps[i].ng, kb[0], kb[1], kb[2], ldH[q], ldS[q],
+                                   (int64_t)(intptr_t)dH[q], (int64_t)(intptr_t)dS[q],
+                                   (int64_t)hash_words(kps[i].g, sizeof(int32_t) * 3 * (size_t)kps[i].ng,
+                                                       1469598103934665603ull)});
+        }
+        hsgen_ctx::BatchGraph& bg = ctx->bgraphs[b];
+        if (!(bg.exec && bg.sig == sig)) {
+            if (bg.exec) {
+                HIPCHK(hipGraphExecDestroy(bg.exec));
+                bg.exec = nullptr;
+            }
+            // G lists, concatenated in batch order
+            std::vector<int32_t> gall;
+            std::vector<int64_t> goff(ks.size());
+            for (size_t q = 0; q < ks.size(); ++q) {
+                goff[q] = (int64_t)gall.size();
+                gall.insert(gall.end(), kps[ks[q]].g, kps[ks[q]].g + 3 * (size_t)kps[ks[q]].ng);
+            }
+            st = ensure(ctx, bg.g, sizeof(int32_t) * std::max<size_t>(gall.size(), 3), false);
+            if (st) return st;
+            // problems: T application over all columns, then S and H per k
+            std::vector<GemmProblem> probs;
+            std::vector<TileWork> tw_t, tw_c;
+            if (do_h) {
+                const int ntn = (int)((ncols + geo.t.bn - 1) / geo.t.bn);
+                for (int a = 0; a < s->natoms; ++a) {
+                    const AtomDev& at = s->atoms[a];
+                    GemmProblem p;
+                    p.L = s->d_M + s->m_off[a];
+                    p.ldl = s->m_ld[a];
+                    p.R = Z + 2 * (int64_t)at.off;
+                    p.ldr = geo.ldz;
+                    p.C = W + 2 * (int64_t)at.off;
+                    p.ldc = geo.ldz;
+                    p.m = 2 * at.nlm;
+                    p.n = (int32_t)ncols;
+                    p.kdepth = s->m_ld[a];
+                    p.lower = 0;
+                    const int id = (int)probs.size();
+                    probs.push_back(p);
+                    const int ntm = (p.m + geo.t.bm - 1) / geo.t.bm;
+                    for (int bi = 0; bi < ntm; ++bi)
+                        for (int bj = 0; bj < ntn; ++bj) tw_t.push_back(TileWork{id, bi, bj, 0});
+                }
+            }
+            for (size_t q = 0; q < ks.size(); ++q) {
+                const int i = ks[q];
+                const size_t p0 = probs.size();
+                add_hs_problems(probs, tw_c, geo, kps[i].ng, Z + col0[q] * geo.ldz, Zs + col0[q] * geo.ldz,
+                                W + col0[q] * geo.ldz, dH[q], dS[q], 0, flags);
+                for (size_t pi = p0; pi < probs.size(); ++pi) {
+                    probs[pi].ldl = geo.ldz;
+                    probs[pi].ldr = geo.ldz;
+                    probs[pi].kdepth = (int32_t)geo.kpad;
+                    probs[pi].ldc = probs[pi].C == dH[q] ? ldH[q] : ldS[q];
+                }
+            }
+            xcd_permute(tw_c);
+            std::vector<TileWork> work(tw_t);
+            work.insert(work.end(), tw_c.begin(), tw_c.end());
+            st = ensure(ctx, bg.probs, sizeof(GemmProblem) * std::max<size_t>(probs.size(), 1), false);
+            if (st) return st;
+            st = ensure(ctx, bg.work, sizeof(TileWork) * std::max<size_t>(work.size(), 1), false);
+            if (st) return st;
+            if (!gall.empty())
+                HIPCHK(hipMemcpyAsync(bg.g.p, gall.data(), sizeof(int32_t) * gall.size(), hipMemcpyHostToDevice,
+                                      ctx->stream));
+            HIPCHK(hipMemcpyAsync(bg.probs.p, probs.data(), sizeof(GemmProblem) * probs.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+            if (!work.empty())
+                HIPCHK(hipMemcpyAsync(bg.work.p, work.data(), sizeof(TileWork) * work.size(),
+                                      hipMemcpyHostToDevice, ctx->stream));
+            // the host vectors must outlive the copies
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            // capture the launch sequence
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipSuccess;
+            for (size_t q = 0; q < ks.size() && ce == hipSuccess; ++q) {
+                const int i = ks[q];
+                AbgenParams p = abgen_params(s, &kps[i], (const int32_t*)bg.g.p + goff[q]);
+                p.ncols = (int32_t)roundup(std::max(kps[i].ng, 1), tm);
+                p.Z = Z + col0[q] * geo.ldz;
+                p.Zs = Zs + col0[q] * geo.ldz;
+                p.ldz = geo.ldz;
+                ce = launch_abgen(p, true, ctx->stream);
+            }
+            const GemmProblem* dp = (const GemmProblem*)bg.probs.p;
+            const TileWork* dw = (const TileWork*)bg.work.p;
+            if (ce == hipSuccess && do_h) {
+                ce = launch_zgemm(geo.cfg, dp, dw, (int)tw_t.size(), ctx->stream, pgrid);
+                if (ce == hipSuccess)
+                    ce = launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)ncols, ctx->stream);
+            }
+            if (ce == hipSuccess)
+                ce = launch_zgemm(geo.cfg, dp, dw + tw_t.size(), (int)tw_c.size(), ctx->stream, pgrid);
+            for (size_t q = 0; q < ks.size() && ce == hipSuccess && (flags & HSGEN_FILL_UPPER); ++q) {
+                if (do_h) ce = launch_fill_upper(dH[q], ldH[q], kps[ks[q]].ng, ctx->stream);
+                if (ce == hipSuccess && do_s) ce = launch_fill_upper(dS[q], ldS[q], kps[ks[q]].ng, ctx->stream);
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(ctx, HSGEN_ERR_HIP, "graph capture failed: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            const hipError_t ie = hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                bg.exec = nullptr;
+                return fail(ctx, HSGEN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+            }
+            bg.sig = sig;
+        }
+        HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+        HIPCHK(hipGraphLaunch(bg.exec, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+        HIPCHK(hipEventSynchronize(ctx->ev[1]));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ms_all += ms;
+        for (size_t q = 0; q < ks.size(); ++q) {
+            const int i = ks[q];
+            const bool full = (flags & HSGEN_FILL_UPPER) != 0;
+            if (do_h && !hdev[i]) {
+                st = copy_lower_to_host(ctx, dH[q], kps[i].ng, H[i], ld[i], full);
+                if (st) return st;
+            }
+            if (do_s && !sdev[i]) {
+                st = copy_lower_to_host(ctx, dS[q], kps[i].ng, S[i], ld[i], full);
+                if (st) return st;
+            }
+        }
+    }
+    if (rep) {
+        for (int i = 0; i < nk; ++i) {
+            hsgen_report& r = rep[i];
+            std::memset(&r, 0, sizeof r);
+            r.ng = kps[i].ng;
+            r.kdim = s->kdim;
+            r.flops_alg = 16.0 * (double)kps[i].ng * kps[i].ng * s->kdim + 32.0 * (double)kps[i].ng * (double)s->sum_nlm2;
+            r.bytes_abgen = 2.0 * 2.0 * 16.0 * (double)kps[i].ng * s->kdim;
+            r.tile_used = geo.cfg;
+        }
+        if (nk > 0) rep[0].ms_total = ms_all;
     }
     return HSGEN_OK;
 }
@@ -1219,6 +1530,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
         return fail(ctx, HSGEN_ERR_ARG, "slices %d outside [0, %d]", o.slices, kMaxModuli);
     const int ng = kp->ng;
     if (ld < std::max(ng, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld %lld < ng %d", (long long)ld, ng);
+    o.tile = pick_tile(ctx, o, ng, s->kdim);
     if ((!H && !(o.flags & HSGEN_SKIP_H)) || (!S && !(o.flags & HSGEN_SKIP_S)))
         return fail(ctx, HSGEN_ERR_ARG, "null output matrix");
     if (rep) std::memset(rep, 0, sizeof *rep);
@@ -1260,10 +1572,88 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32
 {
     if (!ctx || !sys || nk < 0 || (nk > 0 && (!kps || !H || !S || !ld)))
         return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    hsgen_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
+        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    for (int i = 0; i < nk; ++i) {
+        hsgen_status st = check_kpoint(ctx, &kps[i]);
+        if (st) return st;
+        if (ld[i] < std::max(kps[i].ng, 1))
+            return fail(ctx, HSGEN_ERR_ARG, "k-point %d: ld %lld < ng %d", i, (long long)ld[i], kps[i].ng);
+        if ((!H[i] && !(o.flags & HSGEN_SKIP_H)) || (!S[i] && !(o.flags & HSGEN_SKIP_S)))
+            return fail(ctx, HSGEN_ERR_ARG, "k-point %d: null output matrix", i);
+    }
+    if (nk == 0) return HSGEN_OK;
+    if (o.precision == HSGEN_PREC_FP64) {
+        HIPCHK(hipSetDevice(ctx->device));
+        int max_ng = 1;
+        for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
+        o.tile = pick_tile(ctx, o, max_ng, sys->kdim);
+        return batched_fp64(ctx, const_cast<hsgen_sys*>(sys), nk, kps, o, H, S, ld, rep);
+    }
+    // split precision: the modulus count is chosen per k from its own error
+    // model, so the k-points run one after the other
+    double ms = 0.0;
     for (int i = 0; i < nk; ++i) {
         hsgen_status st = hsgen_create_hs(ctx, sys, &kps[i], opts, H[i], S[i], ld[i], rep ? &rep[i] : nullptr);
         if (st) return st;
+        if (rep) ms += rep[i].ms_total;
     }
+    if (rep) rep[0].ms_total = ms;
+    return HSGEN_OK;
+}
+
+hsgen_status hsgen_autotune(hsgen_ctx* ctx, const hsgen_sys* sys, const hsgen_kpoint* kp, int32_t precision,
+                            int32_t* best_tile, double* best_ms)
+{
+    if (!ctx || !sys) return fail(ctx, HSGEN_ERR_ARG, "null argument");
+    hsgen_status st = check_kpoint(ctx, kp);
+    if (st) return st;
+    if (precision != HSGEN_PREC_FP64 && precision != HSGEN_PREC_OZAKI_INT8)
+        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", precision);
+    const int ng = kp->ng;
+    if (ng <= 0) return fail(ctx, HSGEN_ERR_ARG, "autotuning needs ng > 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<int> cands;
+    if (precision == HSGEN_PREC_FP64) {
+        for (int id = 1; id < 64; ++id)
+            if (zgemm_cfg(id).bm > 0) cands.push_back(id);
+    } else {
+        for (int id = 1; id < 64; ++id)
+            if (igemm_cfg(id).bm > 0) cands.push_back(id);
+    }
+    // device outputs owned by the call
+    st = ensure(ctx, ctx->Htmp, sizeof(double2) * (size_t)ng * ng, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->Stmp, sizeof(double2) * (size_t)ng * ng, false);
+    if (st) return st;
+    double* dH = (double*)ctx->Htmp.p;
+    double* dS = (double*)ctx->Stmp.p;
+    int best = 0;
+    double bms = 0.0;
+    for (int id : cands) {
+        hsgen_opts o{};
+        o.precision = precision;
+        o.tile = id;
+        // split precision: a fixed modulus count; the INT8 GEMM time per
+        // modulus is what the configuration changes
+        o.slices = precision == HSGEN_PREC_FP64 ? 0 : 10;
+        hsgen_report r{};
+        double ms = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {   // warm-up, then the timed run
+            st = hsgen_create_hs(ctx, sys, kp, &o, dH, dS, ng, &r);
+            if (st) return st;
+            ms = precision == HSGEN_PREC_FP64 ? r.ms_contract : r.ms_igemm;
+        }
+        if (best == 0 || ms < bms) {
+            best = id;
+            bms = ms;
+        }
+    }
+    ctx->tuned[tune_key(precision, ng, sys->kdim)] = best;
+    if (best_tile) *best_tile = best;
+    if (best_ms) *best_ms = bms;
     return HSGEN_OK;
 }
 
@@ -1318,6 +1708,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     }
     s.kdim = off;
     if (G == 0) return HSGEN_OK;
+    o.tile = pick_tile(ctx, o, G, s.kdim);
     const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
                                   o.precision == HSGEN_PREC_FP64 ? 1 : 256);
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);

@@ -93,7 +93,8 @@ struct ZgemmLaunchCfg {
 };
 ZgemmLaunchCfg zgemm_cfg(int id);
 hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
-                        hipStream_t st);
+                        hipStream_t st, int max_grid = 0);
+hipError_t launch_clear_rows(double2* X, int64_t ld, int64_t r0, int64_t r1, int ncols, hipStream_t st);
 hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st);
 
 // ---- split precision (ozaki.hip) ----
@@ -158,7 +159,9 @@ hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int
                          hipStream_t st);
 hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st);
 hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
-                            int j0, int lower, double* part, hipStream_t st);
+                            const int32_t* cols, int lower, double* part, hipStream_t st);
+hipError_t launch_gather_cols(const double2* X, int64_t ld, const int32_t* cols, int rows, int n, double2* Y,
+                              hipStream_t st);
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);
 IgemmLaunchCfg igemm_cfg(int id);
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,

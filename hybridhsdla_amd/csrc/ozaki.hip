@@ -95,25 +95,27 @@ hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part,
     return hipGetLastError();
 }
 
-// Statistics of a column block Y (m x n, ld ldy) whose column j is column
-// j0 + j of a square matrix, one block per column:
+// Statistics of a column sample Y (m x n, ld ldy) whose column j is column
+// cols[j] of a square matrix X, one block per sample column:
 //   part[3 j]     = sum |Y_ij|^2,
-//   part[3 j + 1] = sum |X_i,j0+j - Y_ij|^2   (0 when X is null),
-//   part[3 j + 2] = |Y_j0+j,j|^2              (the diagonal element),
-// over all rows i < m, or over i >= j0 + j when lower != 0.
+//   part[3 j + 1] = sum |X_i,cols[j] - Y_ij|^2   (0 when X is null),
+//   part[3 j + 2] = |Y_cols[j],j|^2               (the diagonal element),
+// over all rows i < m, or over i >= cols[j] when lower != 0.
 __global__ __launch_bounds__(256) void k_block_cmp(const double2* __restrict__ X, int64_t ldx,
                                                    const double2* __restrict__ Y, int64_t ldy, int m,
-                                                   int j0, int lower, double* __restrict__ part)
+                                                   const int32_t* __restrict__ cols, int lower,
+                                                   double* __restrict__ part)
 {
     __shared__ double ry[256], rd[256];
     const int j = blockIdx.x;
-    const int i0 = lower ? j0 + j : 0;
+    const int cj = cols[j];
+    const int i0 = lower ? cj : 0;
     double qy = 0.0, qd = 0.0;
     for (int i = i0 + threadIdx.x; i < m; i += blockDim.x) {
         const double2 y = Y[(int64_t)j * ldy + i];
         qy = fma(y.x, y.x, fma(y.y, y.y, qy));
         if (X) {
-            const double2 x = X[(int64_t)(j0 + j) * ldx + i];
+            const double2 x = X[(int64_t)cj * ldx + i];
             const double dx = x.x - y.x, dy = x.y - y.y;
             qd = fma(dx, dx, fma(dy, dy, qd));
         }
@@ -132,19 +134,38 @@ __global__ __launch_bounds__(256) void k_block_cmp(const double2* __restrict__ X
         part[3 * j] = ry[0];
         part[3 * j + 1] = rd[0];
         double dg = 0.0;
-        if (j0 + j < m) {
-            const double2 y = Y[(int64_t)j * ldy + j0 + j];
+        if (cj < m) {
+            const double2 y = Y[(int64_t)j * ldy + cj];
             dg = y.x * y.x + y.y * y.y;
         }
         part[3 * j + 2] = dg;
     }
 }
 
+// Y[:, j] = X[:, cols[j]] for j < n (rows values per column, same leading
+// dimension ld for both)
+__global__ __launch_bounds__(256) void k_gather_cols(const double2* __restrict__ X, int64_t ld,
+                                                     const int32_t* __restrict__ cols, int rows,
+                                                     double2* __restrict__ Y)
+{
+    const int j = blockIdx.y;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) Y[(int64_t)j * ld + r] = X[(int64_t)cols[j] * ld + r];
+}
+
+hipError_t launch_gather_cols(const double2* X, int64_t ld, const int32_t* cols, int rows, int n, double2* Y,
+                              hipStream_t st)
+{
+    if (n <= 0 || rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_cols, dim3((rows + 255) / 256, n), dim3(256), 0, st, X, ld, cols, rows, Y);
+    return hipGetLastError();
+}
+
 hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
-                            int j0, int lower, double* part, hipStream_t st)
+                            const int32_t* cols, int lower, double* part, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_block_cmp, dim3(n), dim3(256), 0, st, X, ldx, Y, ldy, m, j0, lower, part);
+    hipLaunchKernelGGL(k_block_cmp, dim3(n), dim3(256), 0, st, X, ldx, Y, ldy, m, cols, lower, part);
     return hipGetLastError();
 }
 

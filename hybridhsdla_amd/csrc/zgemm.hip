@@ -42,20 +42,17 @@ struct ZgemmCfg {
     static_assert((BM + BN) * BK % kThreads == 0, "stage must split evenly over threads");
 };
 
+// One output tile (work item wid); lds holds two stages.
 template <int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* __restrict__ probs,
-                                                             const TileWork* __restrict__ work,
-                                                             int nwork)
+__device__ __forceinline__ void zgemm_tile(const GemmProblem* __restrict__ probs,
+                                           const TileWork* __restrict__ work, int wid, double* lds)
 {
     using Cfg = ZgemmCfg<BM, BN, BK, WGM, WGN>;
     constexpr int NT = Cfg::kThreads;
     constexpr int RB = Cfg::kRB, CB = Cfg::kCB;
     constexpr int CS = Cfg::kColStride;
     constexpr int NL = Cfg::kLoads;
-    __shared__ double lds[2 * Cfg::kStageDoubles];
 
-    const int wid = blockIdx.x;
-    if (wid >= nwork) return;
     const TileWork tw = work[wid];
     const GemmProblem pb = probs[tw.prob];
     const int I0 = tw.bi * BM;
@@ -173,6 +170,24 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
         }
 }
 
+// Workgroups take tiles wid = blockIdx.x, blockIdx.x + gridDim.x, ...: with
+// one workgroup per tile (grid = nwork) that is a single tile; with a grid of
+// one workgroup per CU slot the launch is persistent and every workgroup
+// walks the list, keeping the dispatch order (and so the XCD a tile runs on)
+// of the one-per-tile launch.
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* __restrict__ probs,
+                                                             const TileWork* __restrict__ work,
+                                                             int nwork)
+{
+    using Cfg = ZgemmCfg<BM, BN, BK, WGM, WGN>;
+    __shared__ double lds[2 * Cfg::kStageDoubles];
+    for (int wid = blockIdx.x; wid < nwork; wid += gridDim.x) {
+        zgemm_tile<BM, BN, BK, WGM, WGN>(probs, work, wid, lds);
+        __syncthreads();   // the next tile's first stage overwrites LDS
+    }
+}
+
 // Tile configurations (id -> template instance).
 
 #define HSGEN_ZGEMM_CFGS(X)        \
@@ -195,12 +210,13 @@ ZgemmLaunchCfg zgemm_cfg(int id)
 }
 
 hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
-                        hipStream_t st)
+                        hipStream_t st, int max_grid)
 {
     if (nwork <= 0) return hipSuccess;
+    const int grid = max_grid > 0 ? std::min(nwork, max_grid) : nwork;
 #define X(ID, BM, BN, BK, WGM, WGN)                                                      \
     if (id == ID) {                                                                      \
-        hipLaunchKernelGGL((k_zgemm_lh<BM, BN, BK, WGM, WGN>), dim3(nwork),             \
+        hipLaunchKernelGGL((k_zgemm_lh<BM, BN, BK, WGM, WGN>), dim3(grid),              \
                            dim3(WGM * WGN * 64), 0, st, probs, work, nwork);             \
         return hipGetLastError();                                                        \
     }
@@ -218,6 +234,22 @@ __global__ void k_fill_upper(double2* C, int64_t ld, int n)
         double2 v = C[(int64_t)i * ld + j];
         C[(int64_t)j * ld + i] = make_double2(v.x, -v.y);
     }
+}
+
+// Zero rows r0 .. r1-1 of ncols columns (leading dimension ld); a kernel
+// rather than a memset so that it can sit in a captured graph.
+__global__ void k_clear_rows(double2* X, int64_t ld, int64_t r0, int64_t r1)
+{
+    const int64_t r = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < r1) X[(int64_t)blockIdx.y * ld + r] = make_double2(0.0, 0.0);
+}
+
+hipError_t launch_clear_rows(double2* X, int64_t ld, int64_t r0, int64_t r1, int ncols, hipStream_t st)
+{
+    if (r1 <= r0 || ncols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_rows, dim3((unsigned)((r1 - r0 + 255) / 256), ncols), dim3(256), 0, st, X, ld,
+                       r0, r1);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st)

@@ -90,11 +90,12 @@ typedef struct {
     int32_t precision;         /* hsgen_precision */
     int32_t slices;            /* split precision: INT8 residue slices per operand, i.e. the
                                   number of moduli (0 = chosen for target_rel_err from an error
-                                  model and an FP64 sample block of H and S, then checked
+                                  model and an FP64 sample of columns of H and S, then checked
                                   against the model with the exact norms, with a retry) */
     double target_rel_err;     /* split precision: target relative Frobenius error (0 = 1e-12) */
     int32_t flags;             /* HSGEN_FILL_UPPER | HSGEN_SKIP_H | HSGEN_SKIP_S */
-    int32_t tile;              /* contraction tile config id (0 = autotuned default) */
+    int32_t tile;              /* contraction tile config id (0 = the one hsgen_autotune chose
+                                  for this size, else the built-in default) */
 } hsgen_opts;
 
 typedef struct {
@@ -113,8 +114,8 @@ typedef struct {
     double err_est_s;          /* ... and of S (0 in FP64) */
     double ms_igemm;           /* split precision: GPU time of the INT8 GEMM launches alone
                                   (last modulus set when an automatic retry ran) */
-    double err_smp_h;          /* split precision: measured relative Frobenius error of H on a
-                                  block of bn columns (lower part) recomputed in FP64 */
+    double err_smp_h;          /* split precision: measured relative Frobenius error of H on
+                                  bn evenly spaced columns (their lower parts) recomputed in FP64 */
     double err_smp_s;          /* ... and of S */
 } hsgen_report;
 
@@ -160,6 +161,17 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys,
                                      const hsgen_opts* opts, double* const* H,
                                      double* const* S, const int64_t* ld,
                                      hsgen_report* report);
+
+/* Autotuner: time every tile configuration of the contraction (FP64 GEMM,
+ * or the INT8 GEMM of split precision at a fixed 10 moduli) on this
+ * k-point and remember the fastest for problems of its size (half-octave
+ * buckets of G and K).  Later calls with opts.tile = 0 use the remembered
+ * configuration; explicit opts.tile values still win.  No reference
+ * counterpart: it takes the place of the reference's per-machine GPU/CPU
+ * split-ratio tuning (compute_ratio.cpp, wrappers.cpp:73-109).  *best_tile
+ * and *best_ms (contraction GPU time of the winner) may be NULL. */
+hsgen_status hsgen_autotune(hsgen_ctx* ctx, const hsgen_sys* sys, const hsgen_kpoint* kp,
+                            int32_t precision, int32_t* best_tile, double* best_ms);
 
 /* Contraction only, from precomputed A and B.  Replaces
  * create_HS2_stripped_v2 (flapw_opt/src/include/acc/h_and_s_stripped.h:24-34,

@@ -15,6 +15,16 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch ships its own HIP runtime under the same soname as the system
+    # one libhsgen.so links; whichever loads first serves the process, and
+    # torch cannot start on the other.  Tests that hand torch device buffers
+    # to the library therefore start torch's runtime first, as bench.py does.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     from hybridhsdla_amd.api import Context
     c = Context(0)
     yield c

@@ -202,3 +202,95 @@ def test_split_precision_tile_configs_bitwise_equal(ctx):
     for t in (2, 3, 4, 5, 6, 11, 12, 13, 14, 15, 16, 21, 22, 23, 24, 25, 26, 27, 28, 29, 31, 32, 33, 34, 35):
         H, S = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=13, tile=t)
         assert np.array_equal(H, H0) and np.array_equal(S, S0), t
+
+
+# ---------------------------------------------------------------------------
+# batched k-points (grouped launches captured in a hipGraph)
+# ---------------------------------------------------------------------------
+def _kset(s, nk=5, seed=3):
+    from hybridhsdla_amd.system import KPoint, gvectors
+    rng = np.random.default_rng(seed)
+    b33 = s.bmat33()
+    kps = []
+    for i in range(nk):
+        k = rng.uniform(-0.5, 0.5, 3)
+        kmax = rng.uniform(2.0, 3.2)
+        kps.append(KPoint(k=k, g=gvectors(k, b33, kmax), kmax=kmax))
+    return kps
+
+
+@pytest.mark.parametrize("budget", [None, "2000000"])
+def test_batched_equals_single_k(ctx, monkeypatch, budget):
+    """All k-points in one grouped launch (or, with a small memory cap,
+    several batches) give the single-k results bit for bit (same tiles, same
+    summation order) and the oracle's to 1e-13."""
+    if budget:
+        monkeypatch.setenv("HSGEN_BATCH_BYTES", budget)
+    s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
+    kps = _kset(s)
+    print("batched ng:", [kp.ng for kp in kps], "budget", budget, flush=True)
+    h = ctx.upload(s)
+    res, reps = ctx.create_HS2_batched(h, kps)
+    for kp, (H, S), r in zip(kps, res, reps):
+        H1, S1 = ctx.create_HS2(h, kp)
+        assert np.array_equal(H, H1) and np.array_equal(S, S1)
+        assert r["ng"] == kp.ng
+    Hr, Sr = po.reference_hs(s, kps[0])
+    assert rel_fro_lower(res[0][0], Hr) <= TOL and rel_fro_lower(res[0][1], Sr) <= TOL
+    assert reps[0]["ms_total"] > 0
+
+
+def test_batched_graph_replay_and_rebuild(ctx):
+    """A repeated call replays the captured graph; changing a G list (same
+    sizes) must rebuild it, not replay stale work."""
+    import torch
+    s, _ = tiny_system(natoms=2, lmax=(3, 3), a_cell=7.0)
+    kps = _kset(s, nk=4, seed=5)
+    h = ctx.upload(s)
+    dev = [(torch.zeros((kp.ng, kp.ng), dtype=torch.complex128, device="cuda"),
+            torch.zeros((kp.ng, kp.ng), dtype=torch.complex128, device="cuda")) for kp in kps]
+    outs = [(a.data_ptr(), b.data_ptr(), kp.ng) for (a, b), kp in zip(dev, kps)]
+    ctx.create_HS2_batched(h, kps, outputs=outs)
+    first = [(a.cpu().numpy().T.copy(), b.cpu().numpy().T.copy()) for a, b in dev]
+    for a, b in dev:
+        a.zero_()
+        b.zero_()
+    ctx.create_HS2_batched(h, kps, outputs=outs)              # replay
+    for (a, b), (H0, S0) in zip(dev, first):
+        assert np.array_equal(a.cpu().numpy().T, H0) and np.array_equal(b.cpu().numpy().T, S0)
+    # reverse one G list: same ng, other content -> other H
+    kps[1].g = kps[1].g[::-1].copy()
+    ctx.create_HS2_batched(h, kps, outputs=outs)
+    H1, S1 = ctx.create_HS2(h, kps[1])
+    assert np.array_equal(np.tril(dev[1][0].cpu().numpy().T), H1)
+    assert np.array_equal(np.tril(dev[1][1].cpu().numpy().T), S1)
+
+
+def test_batched_split_precision(ctx):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
+    kps = _kset(s, nk=3, seed=9)
+    h = ctx.upload(s)
+    res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    for kp, (H, S), r in zip(kps, res, reps):
+        Hr, Sr = po.reference_hs(s, kp)
+        assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
+        assert r["slices_used"] > 0
+
+
+def test_autotune_picks_and_applies_a_tile(ctx):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0, ng_target=400)
+    h = ctx.upload(s)
+    best, ms = ctx.autotune(h, kp)
+    print(f"autotune FP64: tile {best} ({ms:.3f} ms); ng {kp.ng}", flush=True)
+    assert best in range(1, 9) and ms > 0
+    H, S, rep = ctx.create_HS2(h, kp, report=True)
+    assert rep["tile_used"] == best
+    Hr, Sr = po.reference_hs(s, kp)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+    bi, msi = ctx.autotune(h, kp, precision=HSGEN_PREC_OZAKI_INT8)
+    print(f"autotune INT8: tile {bi} ({msi:.3f} ms)", flush=True)
+    H, S, rep = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+    assert rep["tile_used"] == bi
+    assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
