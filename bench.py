@@ -312,6 +312,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split-precision comparison run after the FP64 run")
     ap.add_argument("--split-target", type=float, default=1e-12)
+    ap.add_argument("--host-out", action="store_true",
+                    help="also time one H+S with host (numpy) outputs, i.e. including the PCIe copies")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="GiB: run one device-to-device copy of this size first (traffic calibration)")
     args = ap.parse_args(argv)
@@ -409,6 +411,19 @@ def main(argv=None) -> int:
         except Exception as e:  # the check must not hide the measurement
             check = {"error": str(e)}
 
+    host_out = None
+    if args.host_out:
+        import numpy as np
+        Hh = np.zeros((G, G), np.complex128, order="F")
+        Sh = np.zeros((G, G), np.complex128, order="F")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.create_HS2(sysh, kp, precision=prec, slices=args.slices, tile=args.tile, H=Hh, S=Sh)
+        host_out = {"sec_per_hs": round(time.perf_counter() - t1, 4),
+                    "note": "host numpy H, S: lower triangles copied over PCIe"}
+        del Hh, Sh
+        log(f"[bench] host outputs: {host_out}")
+
     split = None
     if prec == HSGEN_PREC_FP64 and not args.no_split:
         try:
@@ -451,6 +466,7 @@ def main(argv=None) -> int:
         "abgen_roofline": abgen,
         "parity_sample": check,
         "split_precision": split,
+        "host_outputs": host_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

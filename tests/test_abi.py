@@ -1,0 +1,46 @@
+"""The C ABI libraries load on a machine without a GPU and export every
+function their headers declare (no compute calls here), and the ctypes
+binding declares each of them."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hsgen_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("hsgen.h", "libhsgen.so"), ("hsgen_io.h", "libhsgen_io.so")])
+def test_library_exports_every_declared_function(header, lib):
+    from hybridhsdla_amd import _lib
+    path = ROOT / "hybridhsdla_amd" / "lib" / lib
+    if not path.exists():
+        pytest.skip(f"{lib} not built")
+    _lib.load()                              # libhsgen.so first (shared C++ runtime)
+    so = ctypes.CDLL(str(path))
+    names = _declared(header)
+    assert len(names) >= (14 if header == "hsgen.h" else 12)
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_ctypes_binding_covers_the_headers():
+    from hybridhsdla_amd import _lib
+    from hybridhsdla_amd import dump
+    bound = {n for n, _, _ in _lib.SIGNATURES} | {n for n, _, _ in dump.IO_SIGNATURES}
+    missing = [n for n in _declared("hsgen.h") + _declared("hsgen_io.h") if n not in bound]
+    assert not missing, missing
+
+
+def test_status_strings_without_gpu():
+    from hybridhsdla_amd import _lib
+    lib = _lib.load()
+    assert lib.hsgen_status_string(0) == b"ok"
+    assert lib.hsgen_status_string(-6) == b"accuracy target unreachable"
+    assert lib.hsgen_last_error(None) == b"null context"
