@@ -312,6 +312,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split-precision comparison run after the FP64 run")
     ap.add_argument("--split-target", type=float, default=1e-12)
+    ap.add_argument("--eig", action="store_true",
+                    help="also time the generalized eigensolve H x = lambda S x of the result")
     ap.add_argument("--host-out", action="store_true",
                     help="also time one H+S with host (numpy) outputs, i.e. including the PCIe copies")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -411,6 +413,15 @@ def main(argv=None) -> int:
         except Exception as e:  # the check must not hide the measurement
             check = {"error": str(e)}
 
+    eig = None
+    if args.eig:
+        try:
+            w, _, ms_eig = ctx.eigensolve(H.data_ptr(), S.data_ptr(), n=G, ld=G)
+            eig = {"n": G, "ms": round(ms_eig, 3), "lowest": float(w[0]), "solver": "rocsolver_zhegvd (eigenvalues)"}
+        except Exception as e:      # a muffin-tin-only S need not be positive definite
+            eig = {"n": G, "error": str(e)}
+        log(f"[bench] eigensolve: {eig}")
+
     host_out = None
     if args.host_out:
         import numpy as np
@@ -467,6 +478,7 @@ def main(argv=None) -> int:
         "parity_sample": check,
         "split_precision": split,
         "host_outputs": host_out,
+        "eigensolve": eig,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -189,6 +189,26 @@ class Context:
                                            C.byref(ms)), self.handle)
         return best.value, ms.value
 
+    def eigensolve(self, H, S, vectors: bool = False, n: int = None, ld: int = None):
+        """Generalized eigenproblem H x = lambda S x from lower-triangle H, S
+        (numpy column-major arrays, or device pointers with n and ld given).
+        Returns (w, V or None, gpu_ms); V is column-major."""
+        if isinstance(H, int):
+            if n is None or ld is None:
+                raise ValueError("device pointers need n and ld")
+            hp, sp = H, S
+        else:
+            H = np.asfortranarray(H, dtype=np.complex128)
+            S = np.asfortranarray(S, dtype=np.complex128)
+            n, ld = H.shape[0], H.shape[0]
+            hp, sp = _ptr(H), _ptr(S)
+        w = np.zeros(n)
+        V = np.zeros((n, n), dtype=np.complex128, order="F") if vectors else None
+        ms = C.c_double()
+        _lib.check(self.lib.hsgen_eigensolve(self.handle, n, hp, sp, ld, _ptr(w), _ptr(V) if vectors else None,
+                                             n, C.byref(ms)), self.handle)
+        return w, V, ms.value
+
     def create_HS2_batched(self, sysh: SystemHandle, kps: Sequence[KPoint], precision=HSGEN_PREC_FP64,
                            slices=0, target_rel_err=0.0, flags=0, tile=0, outputs=None):
         """H, S for many k-points; outputs = list of (H_ptr, S_ptr, ld) device

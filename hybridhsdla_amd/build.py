@@ -85,7 +85,8 @@ def build_lib(jobs: int = 4) -> Path:
             _drain(procs)
     _drain(procs)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -22,6 +22,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/hsgen.h"
 #include "hsgen_internal.h"
 
@@ -56,6 +59,9 @@ struct hsgen_ctx {
     DevBuf btmp;                              // device H, S of host-output k-points
     // autotuned tile configurations: (precision, size buckets) -> config id
     std::map<std::vector<int>, int> tuned;
+    // generalized eigensolver
+    rocblas_handle blas = nullptr;
+    DevBuf eig_a, eig_b, eig_d;
 };
 
 struct hsgen_sys {
@@ -383,7 +389,8 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
                       &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp, &ctx->oz_e, &ctx->oz_res,
-                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp, &ctx->btmp})
+                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp, &ctx->btmp, &ctx->eig_a,
+                      &ctx->eig_b, &ctx->eig_d})
         release(*b);
     for (auto& bg : ctx->bgraphs) {
         if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
@@ -405,6 +412,7 @@ hsgen_status hsgen_finalize(hsgen_ctx* ctx)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : ctx->oz_ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HSGEN_OK;
@@ -1771,6 +1779,68 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
         HIPCHK(hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[1]));
         rep->ms_abgen = t0;
         rep->ms_total += t0;
+    }
+    return HSGEN_OK;
+}
+
+
+// The step after H/S generation: the generalized Hermitian eigenproblem
+// H x = lambda S x (Fleur hands H and S to its own solvers, eigen.F:750,781;
+// the reference stops at H and S).  A library call rather than a hand-written
+// kernel: rocSOLVER's zhegvd (Cholesky of S, reduction to a standard problem,
+// divide and conquer) on the context's stream, on copies of the lower
+// triangles hsgen writes.
+hsgen_status hsgen_eigensolve(hsgen_ctx* ctx, int32_t n, const double* H, const double* S, int64_t ld, double* w,
+                              double* V, int64_t ldv, double* ms)
+{
+    if (!ctx) return HSGEN_ERR_ARG;
+    if (n < 0 || !H || !S || !w) return fail(ctx, HSGEN_ERR_ARG, "null argument or n < 0");
+    if (ld < std::max(n, 1) || (V && ldv < std::max(n, 1)))
+        return fail(ctx, HSGEN_ERR_ARG, "leading dimension smaller than n = %d", n);
+    if (n == 0) return HSGEN_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)n * n;
+    hsgen_status st = ensure(ctx, ctx->eig_a, sizeof(double2) * nn, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->eig_b, sizeof(double2) * nn, false);
+    if (st) return st;
+    st = ensure(ctx, ctx->eig_d, sizeof(double) * 2 * (size_t)n + sizeof(rocblas_int), false);
+    if (st) return st;
+    double2* A = (double2*)ctx->eig_a.p;
+    double2* B = (double2*)ctx->eig_b.p;
+    double* D = (double*)ctx->eig_d.p;
+    double* E = D + n;
+    rocblas_int* info = (rocblas_int*)(E + n);
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+            ctx->blas = nullptr;
+            return fail(ctx, HSGEN_ERR_HIP, "rocblas_create_handle failed");
+        }
+    }
+    if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+        return fail(ctx, HSGEN_ERR_HIP, "rocblas_set_stream failed");
+    const size_t row = sizeof(double2) * (size_t)n;
+    HIPCHK(hipMemcpy2DAsync(A, row, H, sizeof(double2) * ld, row, n, hipMemcpyDefault, ctx->stream));
+    HIPCHK(hipMemcpy2DAsync(B, row, S, sizeof(double2) * ld, row, n, hipMemcpyDefault, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    const rocblas_status rs = rocsolver_zhegvd(ctx->blas, rocblas_eform_ax,
+                                               V ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_lower, n,
+                                               (rocblas_double_complex*)A, n, (rocblas_double_complex*)B, n, D, E,
+                                               info);
+    if (rs != rocblas_status_success)
+        return fail(ctx, HSGEN_ERR_HIP, "rocsolver_zhegvd: %s", rocblas_status_to_string(rs));
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    rocblas_int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(w, D, sizeof(double) * n, hipMemcpyDefault, ctx->stream));
+    if (V) HIPCHK(hipMemcpy2DAsync(V, sizeof(double2) * ldv, A, row, row, n, hipMemcpyDefault, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hinfo > n) return fail(ctx, HSGEN_ERR_ARG, "S is not positive definite (leading minor %d)", (int)(hinfo - n));
+    if (hinfo > 0) return fail(ctx, HSGEN_ERR_ACCURACY, "eigensolver did not converge (info %d)", (int)hinfo);
+    if (ms) {
+        float tm = 0.0f;
+        HIPCHK(hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]));
+        *ms = tm;
     }
     return HSGEN_OK;
 }

@@ -449,6 +449,13 @@ __device__ constexpr int vmcnt_only()
     return (N & 15) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14);
 }
 
+// ... vmcnt <= N and lgkmcnt = 0 (this wave's LDS reads have landed)
+template <int N>
+__device__ constexpr int vmcnt_lgkm0()
+{
+    return (N & 15) | (7 << 4) | (0 << 8) | (((N >> 4) & 3) << 14);
+}
+
 template <int CPC>
 __device__ inline int chunk_swz(int c)
 {
@@ -459,10 +466,18 @@ __device__ inline int chunk_swz(int c)
 // between the two 16-bit halves)
 __device__ inline int rot_pairs(int x)
 {
-    const unsigned u = (unsigned)x;
-    const unsigned hi = (u << 8) & 0xFF00FF00u;                       // Lr -> odd bytes
-    const unsigned lo = (0x01000100u - ((u >> 8) & 0x00FF00FFu)) & 0x00FF00FFu;   // -Li -> even bytes
-    return (int)(hi | lo);
+    // bytes (Lr0, Li0, Lr1, Li1) -> (-Li0, Lr0, -Li1, Lr1): a shift puts Lr in
+    // the odd bytes, then two sub-dword (SDWA) subtractions write 0 - Li into
+    // the even bytes and keep the others (3 VALU operations instead of 5)
+    int d = x << 8;
+    const int z = 0;
+    asm("v_sub_u16_sdwa %0, %1, %2 dst_sel:BYTE_0 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:BYTE_1"
+        : "+v"(d)
+        : "v"(z), "v"(x));
+    asm("v_sub_u16_sdwa %0, %1, %2 dst_sel:BYTE_2 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:BYTE_3"
+        : "+v"(d)
+        : "v"(z), "v"(x));
+    return d;
 }
 
 template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR>
@@ -544,52 +559,8 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     const double pm = (double)md.p;
     const double invp = md.invp;
 
-    for (int s = 0; s < nst; ++s) {
-        // stage s landed (NS-2 younger stages may still be in flight), and
-        // every wave is done with stage s-1, whose buffer is refilled next
-        // (__syncthreads would wait for every LDS-DMA load in flight, i.e. drain
-        // the prefetch ring; each wave instead waits for its own loads of stage
-        // s and the bare barrier publishes them.  The previous stage's LDS reads
-        // completed before their MFMAs issued.)
-        __builtin_amdgcn_s_waitcnt(vmcnt_only<LPW * (NS - 2)>());
-        asm volatile("s_barrier" ::: "memory");
-        issue(min(s + NS - 1, nst - 1), (s + NS - 1) % NS);
-        const char* lb = ilds + (s % NS) * SB;
-#pragma unroll
-        for (int ks = 0; ks < KS / 32; ++ks) {
-            i4v a[RB], ar[RB], b[CB], br[CB];
-#pragma unroll
-            for (int i = 0; i < RB; ++i) {
-                const int c = wm * Cfg::kWM + i * 16 + (lane & 15);
-                a[i] = *(const i4v*)&lb[offA[i] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
-            }
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const int c = BM + wn * Cfg::kWN + j * 16 + (lane & 15);
-                b[j] = *(const i4v*)&lb[offB[j] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
-                if (ROTR & 1) {
-                    const int cr = c + BN;
-                    br[j] = *(const i4v*)&lb[offB[j] + BN * Cfg::kColBytes + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(cr))];
-                }
-            }
-            if (!(ROTR & 1)) {
-#pragma unroll
-                for (int i = 0; i < RB; ++i)
-                    ar[i] = i4v{rot_pairs(a[i][0]), rot_pairs(a[i][1]), rot_pairs(a[i][2]), rot_pairs(a[i][3])};
-            }
-#pragma unroll
-            for (int i = 0; i < RB; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
-#pragma unroll
-            for (int i = 0; i < RB; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    aim[i][j] = (ROTR & 1) ? __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], br[j], aim[i][j], 0, 0, 0)
-                                           : __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
-        }
-        // ROTR & 2: the host keeps kdepth within one INT32-safe chunk
+    // mod-p reduction of the INT32 accumulators after stage s when due
+    auto reduce = [&](int s) {
         if (!(ROTR & 2) && (s + 1) % md.chunk_stages == 0 && s + 1 < nst) {
 #pragma unroll
             for (int i = 0; i < RB; ++i)
@@ -602,6 +573,100 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
                         x = (double)aim[i][j][q];
                         aim[i][j][q] = (int)fma(-floor(x * invp), pm, x);
                     }
+        }
+    };
+    auto read_frags = [&](int stage, int ks, i4v* A, i4v* B, i4v* BR) {
+        const char* lb = ilds + (stage % NS) * SB;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int c = wm * Cfg::kWM + i * 16 + (lane & 15);
+            A[i] = *(const i4v*)&lb[offA[i] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
+        }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int c = BM + wn * Cfg::kWN + j * 16 + (lane & 15);
+            B[j] = *(const i4v*)&lb[offB[j] + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(c))];
+            if (ROTR & 1) {
+                const int cr = c + BN;
+                BR[j] = *(const i4v*)&lb[offB[j] + BN * Cfg::kColBytes + 16 * ((q0 + 4 * ks) ^ chunk_swz<CPC>(cr))];
+            }
+        }
+    };
+    auto mfmas = [&](const i4v* a, const i4v* b, const i4v* br) {
+        i4v ar[RB];
+        if (!(ROTR & 1)) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+                ar[i] = i4v{rot_pairs(a[i][0]), rot_pairs(a[i][1]), rot_pairs(a[i][2]), rot_pairs(a[i][3])};
+        }
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+                are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+                aim[i][j] = (ROTR & 1) ? __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], br[j], aim[i][j], 0, 0, 0)
+                                       : __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
+    };
+
+    if constexpr ((ROTR & 4) == 0) {
+        for (int s = 0; s < nst; ++s) {
+            // stage s landed (NS-2 younger stages may still be in flight), and
+            // every wave is done with stage s-1, whose buffer is refilled next
+            // (__syncthreads would wait for every LDS-DMA load in flight, i.e.
+            // drain the prefetch ring; each wave instead waits for its own
+            // loads of stage s and the bare barrier publishes them.  The
+            // previous stage's LDS reads completed before their MFMAs issued.)
+            __builtin_amdgcn_s_waitcnt(vmcnt_only<LPW * (NS - 2)>());
+            asm volatile("s_barrier" ::: "memory");
+            issue(min(s + NS - 1, nst - 1), (s + NS - 1) % NS);
+#pragma unroll
+            for (int ks = 0; ks < KS / 32; ++ks) {
+                i4v a[RB], b[CB], br[CB];
+                read_frags(s, ks, a, b, br);
+                mfmas(a, b, br);
+            }
+            reduce(s);
+        }
+    } else {
+        // Fragments are read one k-step ahead: the reads of the next k-step
+        // (across the stage boundary: after the barrier that publishes the
+        // next stage) go out before this k-step's MFMAs, so the matrix pipe
+        // keeps the MFMAs queued before the barrier busy while waves meet.
+        constexpr int NKS = KS / 32;
+        __builtin_amdgcn_s_waitcnt(vmcnt_only<LPW * (NS - 2)>());
+        asm volatile("s_barrier" ::: "memory");
+        issue(min(NS - 1, nst - 1), (NS - 1) % NS);
+        i4v fa[RB], fb[CB], fbr[CB];
+        read_frags(0, 0, fa, fb, fbr);
+        for (int s = 0; s < nst; ++s) {
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                i4v a[RB], b[CB], br[CB];
+#pragma unroll
+                for (int i = 0; i < RB; ++i) a[i] = fa[i];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    b[j] = fb[j];
+                    br[j] = fbr[j];
+                }
+                if (ks + 1 < NKS) {
+                    read_frags(s, ks + 1, fa, fb, fbr);
+                } else if (s + 1 < nst) {
+                    // stage s+1 landed (NS-2 younger stages in flight) and this
+                    // wave's reads of stage s are complete; after the barrier no
+                    // wave reads stage s, whose buffer takes stage s+NS
+                    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0<LPW * (NS - 2)>());
+                    asm volatile("s_barrier" ::: "memory");
+                    issue(min(s + NS, nst - 1), (s + NS) % NS);
+                    read_frags(s + 1, 0, fa, fb, fbr);
+                }
+                mfmas(a, b, br);
+            }
+            reduce(s);
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -662,7 +727,16 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     Y(32, 128, 256, 32, 2, 2, 4, 2)      \
     Y(33, 256, 128, 64, 4, 1, 3, 2)      \
     Y(34, 256, 128, 64, 4, 2, 3, 2)      \
-    Y(35, 256, 128, 32, 4, 2, 4, 2)
+    Y(35, 256, 128, 32, 4, 2, 4, 2)      \
+    Y(41, 256, 128, 64, 4, 2, 3, 4)      \
+    Y(42, 256, 128, 32, 4, 2, 4, 4)      \
+    Y(43, 128, 128, 64, 2, 2, 4, 4)      \
+    Y(44, 128, 128, 64, 2, 4, 4, 4)      \
+    Y(45, 128, 256, 64, 2, 4, 3, 4)      \
+    Y(46, 256, 128, 32, 4, 2, 5, 4)      \
+    Y(47, 256, 128, 32, 4, 2, 6, 0)      \
+    Y(48, 256, 128, 32, 4, 2, 6, 4)      \
+    Y(49, 128, 256, 32, 2, 4, 6, 0)
 
 IgemmLaunchCfg igemm_cfg(int id)
 {

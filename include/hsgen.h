@@ -196,6 +196,17 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G,
                                const hsgen_opts* opts, double* H, double* S,
                                int64_t ld, hsgen_report* report);
 
+/* The step after H/S: the generalized Hermitian eigenproblem H x = lambda S x
+ * of one k-point.  No counterpart in the reference, which stops at H and S;
+ * Fleur hands them to its own solvers (eigen.F:750,781).  H and S: n x n
+ * lower triangles with leading dimension ld (host or device; not modified).
+ * w[n] receives the eigenvalues in ascending order (host or device); V, if
+ * not NULL, the S-orthonormal eigenvectors (n x n, ldv; host or device).
+ * rocSOLVER's zhegvd on the context's stream; *ms (may be NULL) is its GPU
+ * time.  HSGEN_ERR_ARG if S is not positive definite. */
+hsgen_status hsgen_eigensolve(hsgen_ctx* ctx, int32_t n, const double* H, const double* S, int64_t ld,
+                              double* w, double* V, int64_t ldv, double* ms);
+
 /* ---------------------------------------------------------------------------
  * System set-up on the host (no GPU needed): what turns a Fleur dump into the
  * hsgen_system above.
