@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -62,6 +63,9 @@ struct hsgen_ctx {
     // generalized eigensolver
     rocblas_handle blas = nullptr;
     DevBuf eig_a, eig_b, eig_d;
+    // pinned staging for device -> host copies of H and S
+    void* stage[2] = {nullptr, nullptr};
+    size_t stage_bytes = 0;
 };
 
 struct hsgen_sys {
@@ -79,6 +83,10 @@ struct hsgen_sys {
     RadDev* d_rad = nullptr;
     double* d_ynorm = nullptr;
     double2* d_M = nullptr;
+    // host copies of the T blocks, so that M can be rebuilt for another tile
+    // geometry
+    std::vector<std::vector<std::complex<double>>> h_taa, h_tab, h_tbb;
+    std::vector<int> h_owner;          // per atom: the atom whose copies hold its T
 };
 
 namespace {
@@ -250,23 +258,17 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
     return HSGEN_OK;
 }
 
-// Keep the copy of T pointers so M can be rebuilt for another tile geometry.
-struct SysExtra {
-    std::vector<std::vector<std::complex<double>>> taa, tab, tbb;
-};
-std::map<const hsgen_sys*, SysExtra> g_extra;
-
 hsgen_status ensure_M(hsgen_ctx* ctx, hsgen_sys* s, int bk, int bm)
 {
     if (s->d_M && s->m_bk == bk && s->m_bm == bm) return HSGEN_OK;
-    SysExtra& x = g_extra[s];
     const int na = s->natoms;
     std::vector<const double*> pa(na), pb(na), pc(na);
     std::vector<int32_t> nlm(na);
     for (int a = 0; a < na; ++a) {
-        pa[a] = (const double*)x.taa[a].data();
-        pb[a] = (const double*)x.tab[a].data();
-        pc[a] = (const double*)x.tbb[a].data();
+        const int o = s->h_owner[a];   // atoms sharing T share the pointers (upload_M deduplicates)
+        pa[a] = (const double*)s->h_taa[o].data();
+        pb[a] = (const double*)s->h_tab[o].data();
+        pc[a] = (const double*)s->h_tbb[o].data();
         nlm[a] = s->atoms[a].nlm;
     }
     return upload_M(ctx, s, nlm.data(), pa.data(), pb.data(), pc.data(), bk, bm);
@@ -413,6 +415,8 @@ hsgen_status hsgen_finalize(hsgen_ctx* ctx)
     for (auto& ev : ctx->oz_ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    for (void*& p : ctx->stage)
+        if (p) (void)hipHostFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HSGEN_OK;
@@ -501,10 +505,13 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
         return fail(ctx, HSGEN_ERR_HIP, "system upload: %s", hipGetErrorString(e));
     }
     // keep host copies of T (deduplicated later in upload_M)
-    SysExtra& x = g_extra[s];
-    x.taa.resize(d->natoms);
-    x.tab.resize(d->natoms);
-    x.tbb.resize(d->natoms);
+    auto& xtaa = s->h_taa;
+    auto& xtab = s->h_tab;
+    auto& xtbb = s->h_tbb;
+    xtaa.resize(d->natoms);
+    xtab.resize(d->natoms);
+    xtbb.resize(d->natoms);
+    s->h_owner.assign(d->natoms, 0);
     std::map<const double*, int> seen;
     for (int a = 0; a < d->natoms; ++a) {
         const size_t n2 = (size_t)s->atoms[a].nlm * s->atoms[a].nlm;
@@ -514,16 +521,15 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
         auto it = seen.find(d->t_aa[a]);
         if (it != seen.end() && d->t_ab[a] == d->t_ab[it->second] && d->t_bb[a] == d->t_bb[it->second] &&
             s->atoms[a].nlm == s->atoms[it->second].nlm) {
-            // share storage with the first atom of this type
-            x.taa[a] = x.taa[it->second];
-            x.tab[a] = x.tab[it->second];
-            x.tbb[a] = x.tbb[it->second];
+            // same blocks as an earlier atom (same type): keep one copy
+            s->h_owner[a] = it->second;
             continue;
         }
         seen[d->t_aa[a]] = a;
-        cp(x.taa[a], d->t_aa[a]);
-        cp(x.tab[a], d->t_ab[a]);
-        cp(x.tbb[a], d->t_bb[a]);
+        s->h_owner[a] = a;
+        cp(xtaa[a], d->t_aa[a]);
+        cp(xtab[a], d->t_ab[a]);
+        cp(xtbb[a], d->t_bb[a]);
     }
     Geometry geo = geometry(0, s->kdim, 1);
     st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
@@ -543,7 +549,6 @@ hsgen_status hsgen_system_destroy(hsgen_ctx* ctx, hsgen_sys* s)
     if (s->d_rad) (void)hipFree(s->d_rad);
     if (s->d_ynorm) (void)hipFree(s->d_ynorm);
     if (s->d_M) (void)hipFree(s->d_M);
-    g_extra.erase(s);
     delete s;
     return HSGEN_OK;
 }
@@ -641,19 +646,68 @@ hsgen_status run_grouped(hsgen_ctx* ctx, int cfg, std::vector<GemmProblem>& prob
 }
 
 // Copy the lower triangle (and optionally everything) of a device matrix with
-// ld = n into a host matrix with leading dimension ld.
+// ld = n into a host matrix with leading dimension ld: blocks of columns go
+// through two pinned staging buffers (the DMA of one block overlaps the
+// host-side copy of the previous one, which several threads share), so that
+// only the requested triangle of the caller's matrix is written.
 hsgen_status copy_lower_to_host(hsgen_ctx* ctx, const double2* dsrc, int n, double* H, int64_t ld,
                                 bool full)
 {
-    std::vector<double2> tmp((size_t)n * n);
-    HIPCHK(hipMemcpyAsync(tmp.data(), dsrc, sizeof(double2) * tmp.size(), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    double2* out = (double2*)H;
-    for (int j = 0; j < n; ++j) {
-        const int i0 = full ? 0 : j;
-        std::memcpy(out + (int64_t)j * ld + i0, tmp.data() + (size_t)j * n + i0,
-                    sizeof(double2) * (size_t)(n - i0));
+    const size_t kStage = (size_t)64 << 20;
+    if (!ctx->stage[0]) {
+        for (int b = 0; b < 2; ++b) {
+            hipError_t e = hipHostMalloc(&ctx->stage[b], kStage, hipHostMallocDefault);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->stage[b] = nullptr;
+                return fail(ctx, HSGEN_ERR_NOMEM, "hipHostMalloc(%zu bytes) failed: %s", kStage, hipGetErrorString(e));
+            }
+        }
+        ctx->stage_bytes = kStage;
+    }
+    const size_t colb = sizeof(double2) * (size_t)n;
+    const int cpb = (int)std::max<size_t>(1, ctx->stage_bytes / colb);     // columns per block
+    if ((size_t)cpb * colb > ctx->stage_bytes) {
+        // one column exceeds a staging buffer: copy column pieces directly
+        std::vector<double2> tmp((size_t)n);
+        for (int j = 0; j < n; ++j) {
+            const int i0 = full ? 0 : j;
+            HIPCHK(hipMemcpy(tmp.data() + i0, dsrc + (int64_t)j * n + i0, sizeof(double2) * (n - i0),
+                             hipMemcpyDeviceToHost));
+            std::memcpy((double2*)H + (int64_t)j * ld + i0, tmp.data() + i0, sizeof(double2) * (n - i0));
+        }
+        return HSGEN_OK;
+    }
+    const int nblk = (n + cpb - 1) / cpb;
+    auto issue = [&](int b) -> hipError_t {
+        const int j0 = b * cpb, j1 = std::min(n, j0 + cpb);
+        hipError_t e = hipMemcpyAsync(ctx->stage[b & 1], dsrc + (int64_t)j0 * n, colb * (size_t)(j1 - j0),
+                                      hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev[5 + (b & 1)], ctx->stream);
+        return e;
+    };
+    const int nthr = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    HIPCHK(issue(0));
+    for (int b = 0; b < nblk; ++b) {
+        HIPCHK(hipEventSynchronize(ctx->ev[5 + (b & 1)]));
+        if (b + 1 < nblk) HIPCHK(issue(b + 1));
+        const int j0 = b * cpb, j1 = std::min(n, j0 + cpb);
+        const double2* src = (const double2*)ctx->stage[b & 1];
+        auto work = [&](int t) {
+            for (int j = j0 + t; j < j1; j += nthr) {
+                const int i0 = full ? 0 : j;
+                std::memcpy((double2*)H + (int64_t)j * ld + i0, src + (size_t)(j - j0) * n + i0,
+                            sizeof(double2) * (size_t)(n - i0));
+            }
+        };
+        if (nthr > 1 && (size_t)(j1 - j0) * colb >= ((size_t)4 << 20)) {
+            std::vector<std::thread> th;
+            for (int t2 = 1; t2 < nthr; ++t2) th.emplace_back(work, t2);
+            work(0);
+            for (auto& x : th) x.join();
+        } else {
+            for (int t2 = 0; t2 < nthr; ++t2) work(t2);
+        }
     }
     return HSGEN_OK;
 }

@@ -759,13 +759,10 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
 #define X(ID, BM, BN, BK, WGM, WGN)                                                            \
     if (id == ID) {                                                                            \
         constexpr int lds = IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2;                  \
-        static bool attr_set = false;                                                          \
-        if (!attr_set) {                                                                       \
-            hipError_t e = hipFuncSetAttribute((const void*)k_igemm_crt<BM, BN, BK, WGM, WGN>,  \
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-            if (e != hipSuccess) return e;                                                     \
-            attr_set = true;                                                                   \
-        }                                                                                      \
+        /* per launch: the attribute is per device */                                          \
+        hipError_t e = hipFuncSetAttribute((const void*)k_igemm_crt<BM, BN, BK, WGM, WGN>,      \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);     \
+        if (e != hipSuccess) return e;                                                         \
         hipLaunchKernelGGL((k_igemm_crt<BM, BN, BK, WGM, WGN>), dim3(nwork), dim3(WGM * WGN * 64), \
                            lds, st, probs, work, nwork, md);                                   \
         return hipGetLastError();                                                              \
@@ -775,14 +772,10 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
 #define Y(ID, BM, BN, KS, WGM, WGN, NS, ROTR)                                                   \
     if (id == ID) {                                                                              \
         constexpr int lds = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>::kLdsBytes;                \
-        static bool attr_set = false;                                                            \
-        if (!attr_set) {                                                                         \
-            hipError_t e = hipFuncSetAttribute(                                                  \
-                (const void*)k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, ROTR>,                       \
-                hipFuncAttributeMaxDynamicSharedMemorySize, lds);                                \
-            if (e != hipSuccess) return e;                                                       \
-            attr_set = true;                                                                     \
-        }                                                                                        \
+        /* per launch: the attribute is per device */                                            \
+        hipError_t e = hipFuncSetAttribute((const void*)k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, ROTR>, \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);       \
+        if (e != hipSuccess) return e;                                                           \
         hipLaunchKernelGGL((k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, ROTR>), dim3(nwork),           \
                            dim3(WGM * WGN * 64), lds, st, probs, work, nwork, md);               \
         return hipGetLastError();                                                                \
