@@ -185,6 +185,7 @@ def split_run(args, system, G, H, S, step, fp64_ms, F):
             "bound": "mfma", "kernel": "k_igemm2_crt (INT8 residue GEMMs, one launch per modulus)",
             "achieved": round(ach, 1) if ach else None, "peak": INT8_PEAK, "unit": "TOP/s",
             "frac": round(ach / INT8_PEAK, 4) if ach else None,
+            "traffic": (load_traffic("igemm2_crt", f"cfg{args.config}") or {}).get("bytes_per_launch"),
             "algorithmic_ops_per_launch": 16.0 * G * G * system.kdim,
             "avg_launch_ms": round(ms_ig / max(nmod, 1), 3),
         },

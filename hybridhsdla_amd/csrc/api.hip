@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <string>
 #include <thread>
 #include <vector>
@@ -32,6 +33,7 @@
 using namespace hsgen;
 
 static const int kDefaultTile = 8;
+static const int kIgemmCfg = 51;     // default INT8 kernel configuration (256 x 192 tiles)
 
 struct DevBuf {
     void* p = nullptr;
@@ -181,7 +183,15 @@ struct Geometry {
     int64_t gp;     // padded column count
 };
 
-Geometry geometry(int tile, int kdim, int ng, int col_align = 1)
+// column padding for the split-precision path: a multiple of both sides of
+// the INT8 tile (the same operand supplies rows and columns of S and H)
+int64_t int8_col_align(int tile)
+{
+    const IgemmLaunchCfg c = igemm_cfg((tile > 0 && igemm_cfg(tile).bm > 0) ? tile : kIgemmCfg);
+    return std::lcm<int64_t>(c.bm, c.bn);
+}
+
+Geometry geometry(int tile, int kdim, int ng, int64_t col_align = 1)
 {
     Geometry g;
     g.cfg = tile > 0 ? tile : kDefaultTile;
@@ -190,7 +200,7 @@ Geometry geometry(int tile, int kdim, int ng, int col_align = 1)
         g.cfg = kDefaultTile;
         g.t = zgemm_cfg(g.cfg);
     }
-    const int64_t tm = std::max({g.t.bm, g.t.bn, col_align});
+    const int64_t tm = std::max<int64_t>({(int64_t)g.t.bm, (int64_t)g.t.bn, col_align});
     g.kpad = roundup(2 * (int64_t)kdim, g.t.bk);
     g.ldz = g.kpad + g.t.bk;
     g.gp = roundup(std::max(ng, 1), tm);
@@ -719,7 +729,6 @@ hsgen_status copy_lower_to_host(hsgen_ctx* ctx, const double2* dsrc, int n, doub
 // Pairwise-coprime moduli <= 255 (symmetric residues fit INT8), largest first.
 const int kModuli[kMaxModuli] = {255, 254, 253, 251, 247, 241, 239, 233, 229, 227,
                                  223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
-const int kIgemmCfg = 11;
 
 struct OzPlan {
     int nmod = 0;
@@ -1599,7 +1608,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
     const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
-                                  o.precision == HSGEN_PREC_FP64 ? 1 : 256);
+                                  o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
     st = upload_g(ctx, kp);
     if (st) return st;
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
@@ -1772,7 +1781,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     if (G == 0) return HSGEN_OK;
     o.tile = pick_tile(ctx, o, G, s.kdim);
     const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
-                                  o.precision == HSGEN_PREC_FP64 ? 1 : 256);
+                                  o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);
     if (st) return st;
     struct Guard {

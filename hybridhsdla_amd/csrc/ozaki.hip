@@ -736,7 +736,8 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     Y(46, 256, 128, 32, 4, 2, 5, 4)      \
     Y(47, 256, 128, 32, 4, 2, 6, 0)      \
     Y(48, 256, 128, 32, 4, 2, 6, 4)      \
-    Y(49, 128, 256, 32, 2, 4, 6, 0)
+    Y(49, 128, 256, 32, 2, 4, 6, 0)      \
+    Y(51, 256, 192, 64, 4, 2, 2, 0)
 
 IgemmLaunchCfg igemm_cfg(int id)
 {
