@@ -1,0 +1,160 @@
+"""Edge cases on the GPU, against the oracle and through the error paths.
+
+- no G-vectors, one G-vector;
+- sizes on either side of every tile and padding boundary (128, 256, 768),
+  in FP64 and split precision;
+- atoms with lmax = 0 (the reference allows any lmax_a <= max_lmax; Fleur's
+  sphbes stops for a maximum lmax < 1, `sphbes.f:29`, which the library
+  reports as an argument error);
+- repeated G-vectors (equal rows and columns of H and S);
+- batched k-points of ragged sizes, including one with a single G-vector;
+- argument errors come back as status codes with a message, never a crash.
+
+Tolerances as in test_gpu_parity.py: relative Frobenius error of the lower
+triangle <= 1e-13 (FP64) or the requested target (split precision).
+"""
+import numpy as np
+import pytest
+
+from hybridhsdla_amd import tiny_system
+from hybridhsdla_amd.system import KPoint
+from oracle import pyoracle as po
+from tests.helpers import rel_fro_lower
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def _subset(kp, idx):
+    return KPoint(k=kp.k, g=np.ascontiguousarray(kp.g[np.asarray(idx, dtype=np.int64)]), kmax=kp.kmax)
+
+
+@pytest.fixture(scope="module")
+def big_list():
+    # ~800 G-vectors to cut subsets from
+    return tiny_system(natoms=3, lmax=(3, 2, 4), ng_target=800, seed=5)
+
+
+def test_no_g_vectors(ctx, big_list):
+    s, kp = big_list
+    h = ctx.upload(s)
+    H, S = ctx.create_HS2(h, _subset(kp, []))
+    assert H.shape == (0, 0) and S.shape == (0, 0)
+    A, B = ctx.construct_AB(h, _subset(kp, []))
+    assert A.shape == (s.L, s.natoms, 0)
+
+
+@pytest.mark.parametrize("ng", [1, 2, 127, 128, 129, 255, 257, 767, 769])
+def test_ragged_sizes_fp64(ctx, big_list, ng):
+    s, kp = big_list
+    assert kp.ng >= 769
+    sub = _subset(kp, np.arange(ng) * (kp.ng // ng) if ng > 1 else [kp.ng // 2])
+    print(f"ng={sub.ng}", flush=True)
+    h = ctx.upload(s)
+    H, S = ctx.create_HS2(h, sub)
+    Hr, Sr = po.reference_hs(s, sub)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+    assert not np.any(np.triu(H, 1)) and not np.any(np.triu(S, 1))
+    assert np.all(np.diag(H).imag == 0) and np.all(np.diag(S).imag == 0)
+
+
+@pytest.mark.parametrize("ng", [1, 191, 193, 255, 257, 767, 769])
+def test_ragged_sizes_split_precision(ctx, big_list, ng):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = big_list
+    sub = _subset(kp, np.arange(ng) * (kp.ng // ng) if ng > 1 else [3])
+    print(f"ng={sub.ng}", flush=True)
+    h = ctx.upload(s)
+    H, S, rep = ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+    Hr, Sr = po.reference_hs(s, sub)
+    eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
+    print(f"{rep['slices_used']} moduli: H {eh:.2e} S {es:.2e}", flush=True)
+    assert eh <= 1e-12 and es <= 1e-12
+    assert not np.any(np.triu(H, 1)) and not np.any(np.triu(S, 1))
+
+
+def test_lmax_zero_atoms(ctx):
+    s, kp = tiny_system(natoms=4, lmax=(0, 3, 1), ng_target=60, seed=11)
+    assert 0 in s.lmax
+    h = ctx.upload(s)
+    A, B = ctx.construct_AB(h, kp)
+    Ar, Br = po.construct_ab(s, kp)
+    assert np.linalg.norm(A - Ar) <= TOL * np.linalg.norm(Ar)
+    assert np.linalg.norm(B - Br) <= TOL * np.linalg.norm(Br)
+    H, S = ctx.create_HS2(h, kp)
+    Hr, Sr = po.reference_hs(s, kp)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+
+
+def test_max_lmax_zero_is_an_argument_error(ctx):
+    from hybridhsdla_amd.api import HsgenError
+    s, _ = tiny_system(natoms=2, lmax=(0, 0), ng_target=10)
+    with pytest.raises(HsgenError) as e:
+        ctx.upload(s)
+    assert e.value.status == -1 and "max_lmax" in str(e.value)
+
+
+def test_repeated_g_vectors(ctx, big_list):
+    s, kp = big_list
+    idx = [5, 17, 5, 40, 17, 17]
+    sub = _subset(kp, idx)
+    h = ctx.upload(s)
+    H, S = ctx.create_HS2(h, sub, flags=1)          # HSGEN_FILL_UPPER: full Hermitian matrices
+    Hr, Sr = po.reference_hs(s, sub)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+    # rows of one G-vector agree: bit for bit where both elements are computed
+    # lower-triangle elements (columns left of both rows) and on the diagonal;
+    # to rounding where one of them is the Hermitian mirror of another sum
+    for a, b in ((0, 2), (1, 4), (1, 5), (4, 5)):
+        assert np.array_equal(H[a, :a], H[b, :a]) and np.array_equal(S[a, :a], S[b, :a])
+        assert H[a, a] == H[b, b] and S[a, a] == S[b, b]
+        for M in (H, S):
+            assert np.abs(M[a] - M[b]).max() <= 1e-14 * np.abs(M).max()
+            assert np.abs(M[:, a] - M[:, b]).max() <= 1e-14 * np.abs(M).max()
+
+
+def test_batched_ragged_sizes_equal_single_calls(ctx, big_list):
+    s, kp = big_list
+    kps = [_subset(kp, [7]), _subset(kp, np.arange(129)), _subset(kp, np.arange(300) * 2),
+           _subset(kp, np.arange(2) + 100)]
+    h = ctx.upload(s)
+    res, _ = ctx.create_HS2_batched(h, kps)
+    for (Hb, Sb), k in zip(res, kps):
+        H1, S1 = ctx.create_HS2(h, k)
+        assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
+
+
+def test_argument_errors(ctx, big_list):
+    from hybridhsdla_amd import _lib
+    from hybridhsdla_amd.api import HsgenError, HSGEN_PREC_OZAKI_INT8
+    s, kp = big_list
+    sub = _subset(kp, np.arange(50))
+    h = ctx.upload(s)
+    H = np.zeros((50, 50), np.complex128, order="F")
+    S = np.zeros_like(H)
+    # leading dimension below ng
+    with pytest.raises(HsgenError) as e:
+        ctx.create_hs_device(h, sub, H.ctypes.data, S.ctypes.data, 49)
+    assert e.value.status == -1 and "ld" in str(e.value)
+    # unknown precision
+    with pytest.raises(HsgenError) as e:
+        ctx.create_HS2(h, sub, precision=7)
+    assert e.value.status == -1
+    # more moduli than the table holds
+    with pytest.raises(HsgenError) as e:
+        ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, slices=99)
+    assert e.value.status == -1
+    # too few moduli for the contraction depth: an accuracy error, not a wrong result
+    with pytest.raises(HsgenError) as e:
+        ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, slices=2)
+    assert e.value.status == -6
+    # null outputs
+    lib = _lib.load()
+    with pytest.raises(HsgenError):
+        ctx.create_hs_device(h, sub, 0, 0, 50)
+    # the context still works after the errors
+    H2, S2 = ctx.create_HS2(h, sub)
+    Hr, Sr = po.reference_hs(s, sub)
+    assert rel_fro_lower(H2, Hr) <= TOL and rel_fro_lower(S2, Sr) <= TOL
+    assert lib.hsgen_status_string(-6) == b"accuracy target unreachable"
