@@ -158,3 +158,40 @@ def test_argument_errors(ctx, big_list):
     Hr, Sr = po.reference_hs(s, sub)
     assert rel_fro_lower(H2, Hr) <= TOL and rel_fro_lower(S2, Sr) <= TOL
     assert lib.hsgen_status_string(-6) == b"accuracy target unreachable"
+
+
+def test_contexts_in_parallel_threads(big_list):
+    """Each context owns its stream and workspaces: two contexts driven from
+    two host threads at once give the same bits as one context alone."""
+    import threading
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_OZAKI_INT8
+    s, kp = big_list
+    subs = [_subset(kp, np.arange(400)), _subset(kp, np.arange(300) * 2 + 1)]
+    with Context(0) as c0:
+        h = c0.upload(s)
+        ref = [c0.create_HS2(h, subs[0]), c0.create_HS2(h, subs[1], precision=HSGEN_PREC_OZAKI_INT8, slices=12)]
+        h.close()
+    out = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            with Context(0) as c:
+                hh = c.upload(s)
+                for _ in range(3):
+                    if i == 0:
+                        out[i] = c.create_HS2(hh, subs[0])
+                    else:
+                        out[i] = c.create_HS2(hh, subs[1], precision=HSGEN_PREC_OZAKI_INT8, slices=12)
+                hh.close()
+        except Exception as e:      # reported below, in the main thread
+            errs.append(e)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not errs, errs
+    for (H, S), (Hr, Sr) in zip(out, ref):
+        assert np.array_equal(H, Hr) and np.array_equal(S, Sr)
