@@ -34,6 +34,7 @@ using namespace hsgen;
 
 static const int kDefaultTile = 8;
 static const int kIgemmCfg = 51;     // default INT8 kernel configuration (256 x 192 tiles)
+static const int kSampleTile = 3;    // FP64 tile of the split-precision sample: 64 x 64, so 64 columns
 
 struct DevBuf {
     void* p = nullptr;
@@ -1027,7 +1028,9 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         return fail(ctx, HSGEN_ERR_ACCURACY, "%d moduli give %d bits per element, depth %lld needs %d..52",
                     nmod_req, nmod_req > kMaxModuli ? 0 : oz_beta(nmod_req), (long long)kd, bmin);
     // per-column norm exponents and squared norms of Z, Zs, W; sample statistics
-    const int nb = std::min(ng, geo.t.bn);
+    // over as many columns as the sample tile is wide (one tile column)
+    const ZgemmLaunchCfg sc = zgemm_cfg(kSampleTile);
+    const int nb = std::min(ng, sc.bn);
     const int kNblk = 1024;
     hsgen_status st = ensure(ctx, ctx->oz_e, (sizeof(int32_t) + sizeof(double)) * 3 * (size_t)gp +
                                                  sizeof(double) * (2 * kNblk + 2 * 3 * (size_t)nb) +
@@ -1055,7 +1058,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     // operand's sampled columns are gathered first: the GEMM reads contiguous
     // columns)
     // (the GEMM reads whole tiles of bn columns: the gathered arrays get bn)
-    const size_t nbp = (size_t)roundup(nb, geo.t.bn);
+    const size_t nbp = (size_t)roundup(nb, sc.bn);
     st = ensure(ctx, ctx->oz_smp, sizeof(double2) * 2 * ((size_t)ng * nb + (size_t)geo.ldz * nbp), false);
     if (st) return st;
     double2* smpS = (double2*)ctx->oz_smp.p;
@@ -1067,7 +1070,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
     {
         std::vector<GemmProblem> probs;
         std::vector<TileWork> work;
-        const int ntr = (ng + geo.t.bm - 1) / geo.t.bm;
+        const int ntr = (ng + sc.bm - 1) / sc.bm;
         auto add = [&](const double2* R, double2* C) {
             GemmProblem p;
             p.L = Z;
@@ -1084,7 +1087,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, i
         };
         if (do_s) add(gathS, smpS);
         if (do_h) add(gathH, smpH);
-        st = run_grouped(ctx, geo.cfg, probs, work);
+        st = run_grouped(ctx, kSampleTile, probs, work);
         if (st) return st;
     }
     if (do_s) HIPCHK(launch_block_cmp(nullptr, 0, smpS, ng, ng, nb, dcols, 0, spart, ctx->stream));
