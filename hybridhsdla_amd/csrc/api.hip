@@ -49,7 +49,7 @@ struct hsgen_ctx {
     // workspace
     DevBuf Z, Zs, W, Htmp, Stmp, A, B, g, probs, work, unorm2, atoms_tmp;
     // split precision
-    DevBuf oz_e, oz_res, oz_dd, oz_probs, oz_work, oz_smp;
+    DevBuf oz_e, oz_res, oz_out, oz_probs, oz_work, oz_smp;
     hipEvent_t oz_ev[2 * kMaxModuli] = {};   // INT8 GEMM start/stop per residue group
     int n_cu = 0;                             // compute units (persistent grids)
     // batched k-points: one captured graph per batch, replayed while the
@@ -402,7 +402,7 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (DevBuf* b : {&ctx->Z, &ctx->Zs, &ctx->W, &ctx->Htmp, &ctx->Stmp, &ctx->A, &ctx->B, &ctx->g,
                       &ctx->probs, &ctx->work, &ctx->unorm2, &ctx->atoms_tmp, &ctx->oz_e, &ctx->oz_res,
-                      &ctx->oz_dd, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp, &ctx->btmp, &ctx->eig_a,
+                      &ctx->oz_out, &ctx->oz_probs, &ctx->oz_work, &ctx->oz_smp, &ctx->btmp, &ctx->eig_a,
                       &ctx->eig_b, &ctx->eig_d})
         release(*b);
     for (auto& bg : ctx->bgraphs) {
@@ -823,8 +823,8 @@ struct OzResult {
     double ms_igemm = 0.0;
 };
 
-// One split-precision evaluation with nmod moduli: residues, INT8 GEMMs with
-// CRT accumulation, final recombination into dS / dH.
+// One split-precision evaluation with nmod moduli: residues, INT8 GEMMs that
+// store C mod p per modulus, final recombination into dS / dH.
 hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, int icfg, int nmod,
                         const int32_t* eZ, const int32_t* eZs, const int32_t* eW, double2* dH,
                         int64_t ldH, double2* dS, int64_t ldS, double* ms_igemm)
@@ -837,14 +837,25 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
     const double2* Z = (const double2*)ctx->Z.p;
     const double2* Zs = (const double2*)ctx->Zs.p;
     const double2* W = (const double2*)ctx->W.p;
-    // double-double accumulators (hi, lo) for S and H, ld = ng
-    const size_t accn = (size_t)ng * ng;
-    hsgen_status st = ensure(ctx, ctx->oz_dd, sizeof(double2) * 4 * accn, false);
+    // tile list (shared by every modulus; same tile order as the FP64 path;
+    // problem ids: S (if any) then H); a tile's slot is its place in the list
+    std::vector<IgemmProblem> probs;
+    std::vector<TileWork> work;
+    {
+        std::vector<GemmProblem> gp_dummy;
+        Geometry g2 = geo;
+        g2.t.bm = ic.bm;
+        g2.t.bn = ic.bn;
+        add_hs_problems(gp_dummy, work, g2, ng, nullptr, nullptr, nullptr, nullptr, nullptr, 0, flags);
+        xcd_permute(work);
+        for (size_t w = 0; w < work.size(); ++w) work[w].slot = (int32_t)w;
+    }
+    // residue outputs of every modulus: C mod p_m as 2 bytes per element of
+    // every tile (the recombination reads them all at the end)
+    const size_t mod_stride = 2 * (size_t)ic.bm * ic.bn * work.size();
+    hsgen_status st = ensure(ctx, ctx->oz_out, mod_stride * nmod, false);
     if (st) return st;
-    double2* Shi = (double2*)ctx->oz_dd.p;
-    double2* Slo = Shi + accn;
-    double2* Hhi = Shi + 2 * accn;
-    double2* Hlo = Shi + 3 * accn;
+    uint8_t* rout = (uint8_t*)ctx->oz_out.p;
     // Residue arrays of one modulus ("slot"): Zr, Zs_r, W_r (+ rotated copies
     // the kernel reads), 2 bytes per complex.  As many moduli as fit in 60% of
     // the free memory share one residue pass, which reads the FP64 operands
@@ -865,22 +876,9 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
     st = ensure(ctx, ctx->oz_res, group * slot_bytes, false);
     if (st) return st;
     auto slot = [&](int i, int a) -> int8_t* { return (int8_t*)ctx->oz_res.p + i * slot_bytes + a * arr; };
-    // tile list (shared by every modulus)
-    std::vector<IgemmProblem> probs;
-    std::vector<TileWork> work;
-    {
-        std::vector<GemmProblem> gp_dummy;
-        Geometry g2 = geo;
-        g2.t.bm = ic.bm;
-        g2.t.bn = ic.bn;
-        // same tile order as the FP64 path; problem ids: S (if any) then H
-        add_hs_problems(gp_dummy, work, g2, ng, nullptr, nullptr, nullptr, nullptr, nullptr, 0, flags);
-        xcd_permute(work);
-    }
     // Problem sets, one per (slot, depth chunk).  Kernels without an in-kernel
-    // mod reduction get the depth in INT32-safe chunks, one launch each: every
-    // chunk adds its own residue to the CRT sum (the sum is only needed modulo
-    // 1, so the chunks need not be combined modulo p first).
+    // mod reduction get the depth in INT32-safe chunks, one launch each; the
+    // chunks after the first add their residues mod p to the stored ones.
     const int64_t kc = ic.max_kchunk > 0 ? ic.max_kchunk : kd;
     const int nchunk = (int)((kd + kc - 1) / kc);
     size_t nprob = 0;
@@ -892,8 +890,8 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
         int8_t* Zsrot = ic.needs_rot == 2 ? slot(i, 3) : nullptr;    // (Ri, -Rr) of Zs
         int8_t* Wrot = ic.needs_rot == 2 ? slot(i, 4) : nullptr;     // (Ri, -Rr) of W
         std::vector<IgemmProblem> base;
-        if (do_s) base.push_back(IgemmProblem{Zr, Zrot, Zsr, Zsrot, ldo, ldo, Shi, Slo, ng, ng, ng, (int32_t)kd, 1});
-        if (do_h) base.push_back(IgemmProblem{Zr, Zrot, Wr, Wrot, ldo, ldo, Hhi, Hlo, ng, ng, ng, (int32_t)kd, 1});
+        if (do_s) base.push_back(IgemmProblem{Zr, Zrot, Zsr, Zsrot, ldo, ldo, ng, ng, (int32_t)kd, 1});
+        if (do_h) base.push_back(IgemmProblem{Zr, Zrot, Wr, Wrot, ldo, ldo, ng, ng, (int32_t)kd, 1});
         nprob = base.size();
         for (int c = 0; c < nchunk; ++c)
             for (IgemmProblem p : base) {
@@ -960,12 +958,11 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
             for (int c = 0; c < nchunk; ++c) {
                 IgemmModulus md;
                 md.p = kModuli[m];
-                md.first = m == 0 && c == 0;
+                md.first_chunk = c == 0;
                 md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
                 md.pad = 0;
                 md.invp = 1.0 / kModuli[m];
-                md.whi = pl.whi[m];
-                md.wlo = pl.wlo[m];
+                md.out = rout + (size_t)m * mod_stride;
                 HIPCHK(launch_igemm(icfg, (const IgemmProblem*)ctx->oz_probs.p + (i * nchunk + c) * nprob,
                                     (const TileWork*)ctx->oz_work.p, (int)work.size(), md, ctx->stream));
             }
@@ -973,29 +970,25 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
         HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups + 1], ctx->stream));
     }
     OzFinalParams fp;
-    fp.ldacc = ng;
+    std::memset(&fp, 0, sizeof fp);
+    fp.res = rout;
+    fp.mod_stride = (int64_t)mod_stride;
+    int nfp = 0;
+    if (do_s) fp.prob[nfp++] = OzFinalProblem{dS, ldS, eZ, eZs, ng, ng};
+    if (do_h) fp.prob[nfp++] = OzFinalProblem{dH, ldH, eZ, eW, ng, ng};
+    fp.nmod = nmod;
     fp.beta = pl.beta;
-    fp.n = ng;
+    fp.bm = ic.bm;
+    fp.bn = ic.bn;
+    fp.wgm = ic.wgm;
+    fp.wgn = ic.wgn;
+    for (int m = 0; m < nmod; ++m) {
+        fp.whi[m] = pl.whi[m];
+        fp.wlo[m] = pl.wlo[m];
+    }
     fp.Phi = pl.Phi;
     fp.Plo = pl.Plo;
-    if (do_s) {
-        fp.Chi = Shi;
-        fp.Clo = Slo;
-        fp.eL = eZ;
-        fp.eR = eZs;
-        fp.C = dS;
-        fp.ldc = ldS;
-        HIPCHK(launch_crt_final(fp, ctx->stream));
-    }
-    if (do_h) {
-        fp.Chi = Hhi;
-        fp.Clo = Hlo;
-        fp.eL = eZ;
-        fp.eR = eW;
-        fp.C = dH;
-        fp.ldc = ldH;
-        HIPCHK(launch_crt_final(fp, ctx->stream));
-    }
+    HIPCHK(launch_crt_final(fp, (const TileWork*)ctx->oz_work.p, (int)work.size(), ctx->stream));
     // the host vectors above must outlive the async copies
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *ms_igemm = 0.0;

@@ -44,7 +44,8 @@ struct GemmProblem {
 };
 
 struct TileWork {
-    int32_t prob, bi, bj, pad;
+    int32_t prob, bi, bj;
+    int32_t slot;               // INT8 path: the tile's block of residue outputs; else 0
 };
 
 // Parameters of the A/B generator (abgen.hip).
@@ -122,37 +123,49 @@ struct IgemmProblem {
     const int8_t* R;            // (Rr, Ri) pairs
     const int8_t* Rrot;         // (Ri, -Rr) pairs (kernels that rotate R), else null
     int64_t ldl, ldr;           // bytes
-    double2* Chi;               // double-double accumulator, complex (re, im) parts
-    double2* Clo;
-    int64_t ldc;                // complex units
     int32_t m, n, kdepth, lower;
 };
 
+// One modulus of an INT8 launch.  The kernel writes C mod p, as (re, im)
+// byte pairs in [0, p), into out: tile slot s owns bytes [2 s BM BN,
+// 2 (s+1) BM BN), in the order the MFMA fragments hold them (element
+// (wave, i, j, q, lane) at (((wave RB + i) CB + j) 4 + q) 64 + lane, see
+// igemm_frag_pos).  Launches after the first depth chunk add to what is there.
 struct IgemmModulus {
     int32_t p;
-    int32_t first;
+    int32_t first_chunk;
     int32_t chunk_stages;
     int32_t pad;
     double invp;
-    double whi, wlo;            // u / p as double-double
+    uint8_t* out;
+};
+
+// Recombination of the residue outputs of all moduli (one tile per block of
+// the INT8 work list): problem prob of a tile writes C of oz_final.prob[prob].
+struct OzFinalProblem {
+    double2* C;
+    int64_t ldc;
+    const int32_t* eL;          // column exponents of the left / right operands
+    const int32_t* eR;
+    int32_t m, n;
 };
 
 struct IgemmLaunchCfg {
     int bm, bn, bk, threads, lds_bytes;
     int needs_rot;              // 1: reads precomputed (-Li, Lr); 2: reads precomputed (Ri, -Rr)
     int max_kchunk;             // > 0: no in-kernel mod reduction, launch at most this depth (complex)
+    int wgm, wgn;               // waves along rows / columns of the tile
 };
 
 struct OzFinalParams {
-    const double2* Chi;
-    const double2* Clo;
-    int64_t ldacc;
-    const int32_t* eL;
-    const int32_t* eR;
-    int32_t beta, n;
+    const uint8_t* res;         // residue outputs of modulus 0; modulus m at res + m mod_stride
+    int64_t mod_stride;         // bytes
+    OzFinalProblem prob[2];
+    int32_t nmod, beta;
+    int32_t bm, bn, wgm, wgn;   // INT8 tile: the layout of a slot
+    double whi[kMaxModuli];     // u_m / p_m as double-double (u_m = (P / p_m)^-1 mod p_m)
+    double wlo[kMaxModuli];
     double Phi, Plo;            // product of the moduli as double-double
-    double2* C;
-    int64_t ldc;
 };
 
 hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int32_t* e, double* ss,
@@ -166,6 +179,6 @@ hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);
 IgemmLaunchCfg igemm_cfg(int id);
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
                         const IgemmModulus& md, hipStream_t st);
-hipError_t launch_crt_final(const OzFinalParams& p, hipStream_t st);
+hipError_t launch_crt_final(const OzFinalParams& p, const TileWork* work, int nwork, hipStream_t st);
 
 }  // namespace hsgen

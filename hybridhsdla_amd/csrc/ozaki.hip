@@ -13,11 +13,13 @@
 //   3. v_mfma_i32_16x16x64_i8 accumulates exactly in INT32; the accumulator is
 //      reduced mod p_m every <= 2080 MFMAs (|products| <= 127^2) so it never
 //      overflows;
-//   4. the epilogue adds r_m * (u_m / p_m) (u_m = (P/p_m)^-1 mod p_m, P the
-//      product of the moduli) to a double-double accumulator per element; the
-//      sum is C'/P modulo 1, exact up to the double-double rounding;
-//   5. the final pass takes the centred fractional part, multiplies by P and
-//      rescales by 2^(e_i + e_j - 2 beta).
+//   4. the epilogue stores r_m = C' mod p_m as one byte per real part, in the
+//      order the MFMA fragments hold the tile (coalesced 128-byte stores);
+//   5. the final pass sums r_m * (u_m / p_m) over the moduli (u_m =
+//      (P/p_m)^-1 mod p_m, P the product of the moduli) in double-double,
+//      reduced mod 1: C'/P modulo 1, exact up to the double-double rounding;
+//      it takes the centred fractional part, multiplies by P and rescales by
+//      2^(e_i + e_j - 2 beta).
 // By Cauchy-Schwarz |C'_ij| <= ||L'_:i|| ||R'_:j|| < 2^(2 beta) (1 + 2^-19), so
 // P > 2^(2 beta + 1.01) suffices whatever the depth; the host sets beta from
 // the number of moduli (beta <= 52 keeps every X' an exact double).
@@ -217,15 +219,19 @@ hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st)
 // ---------------------------------------------------------------------------
 // 3. INT8 MFMA GEMM with CRT epilogue
 // ---------------------------------------------------------------------------
-// double-double helpers
+// double-double helpers.  Error-free transformations: no multiply-add
+// contraction may touch them (hipcc contracts by default, and an a*b + c
+// fused into one rounding breaks the error term they compute).
 __device__ inline void two_sum(double a, double b, double& s, double& e)
 {
+#pragma clang fp contract(off)
     s = a + b;
     double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
 }
 __device__ inline void fast_two_sum(double a, double b, double& s, double& e)
 {
+#pragma clang fp contract(off)
     s = a + b;
     e = b - (s - a);
 }
@@ -233,6 +239,7 @@ __device__ inline void fast_two_sum(double a, double b, double& s, double& e)
 // (hi, lo) <- frac((hi, lo) + r * (whi, wlo)), |frac| <= 1/2
 __device__ inline void dd_accum(double& hi, double& lo, double r, double whi, double wlo, bool first)
 {
+#pragma clang fp contract(off)
     double ph = r * whi;
     double pl = fma(r, whi, -ph);
     pl = fma(r, wlo, pl);
@@ -248,6 +255,49 @@ __device__ inline void dd_accum(double& hi, double& lo, double r, double whi, do
     const double n = rint(s);
     s -= n;                              // exact
     two_sum(s, e, hi, lo);               // s may now be smaller than e
+}
+
+// Position of a tile's element in its block of residue outputs: the order in
+// which wave `wave` holds it (MFMA block (i, j), register q, lane lane), so
+// that each store of a wave writes 128 contiguous bytes.
+template <int RB, int CB>
+__device__ __forceinline__ int igemm_frag_pos(int wave, int i, int j, int q, int lane)
+{
+    return (((wave * RB + i) * CB + j) * 4 + q) * 64 + lane;
+}
+
+// Epilogue of the INT8 kernels: C mod p of the tile as (re, im) bytes in
+// [0, p), stored (first depth chunk) or added mod p to the previous chunks'.
+// lane holds D[4*(lane>>4) + q][lane & 15] of MFMA block (i, j) in register q.
+// Every element of the tile is written, padding included: the recombination
+// reads only the lower-triangle elements inside the matrix.
+template <int RB, int CB, int TILE>
+__device__ __forceinline__ void store_residues(const IgemmModulus& md, int slot, int wave, int lane,
+                                               const i4v (&are)[RB][CB], const i4v (&aim)[RB][CB])
+{
+    uchar2* out = (uchar2*)md.out + (int64_t)slot * TILE;
+    const double pm = (double)md.p;
+    const double invp = md.invp;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double xr = (double)are[i][j][q];
+                double xi = (double)aim[i][j][q];
+                int r = (int)fma(-floor(xr * invp), pm, xr);      // [0, p)
+                int s = (int)fma(-floor(xi * invp), pm, xi);
+                const int e = igemm_frag_pos<RB, CB>(wave, i, j, q, lane);
+                if (!md.first_chunk) {
+                    const uchar2 o = out[e];
+                    r += o.x;
+                    s += o.y;
+                    r -= r >= md.p ? md.p : 0;
+                    s -= s >= md.p ? md.p : 0;
+                }
+                out[e] = make_uchar2((unsigned char)r, (unsigned char)s);
+            }
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN>
@@ -380,33 +430,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm_crt(const IgemmProblem
         __syncthreads();
     }
 
-    // epilogue: residue -> double-double CRT accumulation.
-    // lane holds D[4*(lane>>4) + q][lane & 15] in register q
-    const bool lower = pb.lower != 0;
-    const bool first = md.first != 0;
-#pragma unroll
-    for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-            const int gj = J0 + wn * Cfg::kWN + j * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gi = I0 + wm * Cfg::kWM + i * 16 + 4 * (lane >> 4) + q;
-                if (gi < pb.m && gj < pb.n && (!lower || gi >= gj)) {
-                    double xr = (double)are[i][j][q];
-                    double xi = (double)aim[i][j][q];
-                    xr = fma(-floor(xr * invp), pm, xr);      // [0, p)
-                    xi = fma(-floor(xi * invp), pm, xi);
-                    const int64_t idx = (int64_t)gj * pb.ldc + gi;
-                    double2 h = first ? make_double2(0.0, 0.0) : pb.Chi[idx];
-                    double2 l = first ? make_double2(0.0, 0.0) : pb.Clo[idx];
-                    dd_accum(h.x, l.x, xr, md.whi, md.wlo, first);
-                    dd_accum(h.y, l.y, xi, md.whi, md.wlo, first);
-                    pb.Chi[idx] = h;
-                    pb.Clo[idx] = l;
-                }
-            }
-        }
+    store_residues<RB, CB, BM * BN>(md, tw.slot, wave, lane, are, aim);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,32 +694,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
-
-    const bool lower = pb.lower != 0;
-    const bool first = md.first != 0;
-#pragma unroll
-    for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-            const int gj = J0 + wn * Cfg::kWN + j * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gi = I0 + wm * Cfg::kWM + i * 16 + 4 * (lane >> 4) + q;
-                if (gi < pb.m && gj < pb.n && (!lower || gi >= gj)) {
-                    double xr = (double)are[i][j][q];
-                    double xi = (double)aim[i][j][q];
-                    xr = fma(-floor(xr * invp), pm, xr);
-                    xi = fma(-floor(xi * invp), pm, xi);
-                    const int64_t idx = (int64_t)gj * pb.ldc + gi;
-                    double2 h = first ? make_double2(0.0, 0.0) : pb.Chi[idx];
-                    double2 l = first ? make_double2(0.0, 0.0) : pb.Clo[idx];
-                    dd_accum(h.x, l.x, xr, md.whi, md.wlo, first);
-                    dd_accum(h.y, l.y, xi, md.whi, md.wlo, first);
-                    pb.Chi[idx] = h;
-                    pb.Clo[idx] = l;
-                }
-            }
-        }
+    store_residues<RB, CB, BM * BN>(md, tw.slot, wave, lane, are, aim);
 }
 
 #define HSGEN_IGEMM_CFGS(X)        \
@@ -742,15 +741,16 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
 IgemmLaunchCfg igemm_cfg(int id)
 {
 #define X(ID, BM, BN, BK, WGM, WGN) \
-    if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2, 1, 0};
+    if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2, 1, 0, \
+                                        WGM, WGN};
     HSGEN_IGEMM_CFGS(X)
 #undef X
 #define Y(ID, BM, BN, KS, WGM, WGN, NS, ROTR) \
     if (id == ID) return IgemmLaunchCfg{BM, BN, KS, WGM * WGN * 64, Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>::kLdsBytes, \
-                                        (ROTR & 1) ? 2 : 0, (ROTR & 2) ? 65536 : 0};
+                                        (ROTR & 1) ? 2 : 0, (ROTR & 2) ? 65536 : 0, WGM, WGN};
     HSGEN_IGEMM2_CFGS(Y)
 #undef Y
-    return IgemmLaunchCfg{0, 0, 0, 0, 0, 0, 0};
+    return IgemmLaunchCfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
 
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
@@ -789,41 +789,64 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
 // ---------------------------------------------------------------------------
 // 5. CRT finalisation: C = 2^(eL_i + eR_j - 2 beta) * P * centred_frac(acc)
 // ---------------------------------------------------------------------------
-__global__ void k_crt_final(OzFinalParams p)
+// Recombination, one workgroup per tile of the INT8 work list: for each
+// lower-triangle element inside the matrix, the residues of all moduli in
+// modulus order go into a double-double sum of r_m u_m / p_m reduced mod 1
+// (its centred fraction is C' / P), then C = 2^(eL_i + eR_j - 2 beta) P frac.
+__global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWork* __restrict__ work)
 {
-    const int i = blockIdx.x * 16 + threadIdx.x;
-    const int j = blockIdx.y * 16 + threadIdx.y;
-    if (i >= p.n || j >= p.n || i < j) return;
-    const int64_t a = (int64_t)j * p.ldacc + i;
-    const double2 h = p.Chi[a];
-    const double2 l = p.Clo[a];
-    const int sh = p.eL[i] + p.eR[j] - 2 * p.beta;
-    double out[2];
-    const double hh[2] = {h.x, h.y}, ll[2] = {l.x, l.y};
+#pragma clang fp contract(off)
+    const TileWork tw = work[blockIdx.x];
+    const OzFinalProblem pb = p.prob[tw.prob];
+    const int wm_n = p.bm / p.wgm, wn_n = p.bn / p.wgn;     // wave tile
+    const int RB = wm_n / 16, CB = wn_n / 16;
+    const int tile = p.bm * p.bn;
+    const uchar2* r0 = (const uchar2*)p.res + (int64_t)tw.slot * tile;
+    const int64_t mstride = p.mod_stride / 2;
+    for (int e = threadIdx.x; e < tile; e += blockDim.x) {
+        const int lane = e & 63;
+        int t = e >> 6;
+        const int q = t & 3;
+        t >>= 2;
+        const int j = t % CB;
+        t /= CB;
+        const int i = t % RB;
+        const int wave = t / RB;
+        const int gi = tw.bi * p.bm + (wave % p.wgm) * wm_n + i * 16 + 4 * (lane >> 4) + q;
+        const int gj = tw.bj * p.bn + (wave / p.wgm) * wn_n + j * 16 + (lane & 15);
+        if (gi >= pb.m || gj >= pb.n || gi < gj) continue;
+        double hh[2] = {0.0, 0.0}, ll[2] = {0.0, 0.0};
+        for (int m = 0; m < p.nmod; ++m) {
+            const uchar2 r = r0[m * mstride + e];
+            dd_accum(hh[0], ll[0], (double)r.x, p.whi[m], p.wlo[m], m == 0);
+            dd_accum(hh[1], ll[1], (double)r.y, p.whi[m], p.wlo[m], m == 0);
+        }
+        const int sh = pb.eL[gi] + pb.eR[gj] - 2 * p.beta;
+        double out[2];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        // accumulator already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
-        double fh = hh[c], fl = ll[c];
-        const double n = rint(fh);
-        fh -= n;
-        double s, e;
-        two_sum(fh, fl, s, e);
-        // (s + e) * (Phi + Plo)
-        double v = s * p.Phi;
-        double err = fma(s, p.Phi, -v);
-        err = fma(s, p.Plo, err);
-        err = fma(e, p.Phi, err);
-        out[c] = ldexp(v + err, sh);
+        for (int c = 0; c < 2; ++c) {
+            // the sum is already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
+            double fh = hh[c], fl = ll[c];
+            const double n = rint(fh);
+            fh -= n;
+            double s, err0;
+            two_sum(fh, fl, s, err0);
+            // (s + err0) * (Phi + Plo)
+            const double v = s * p.Phi;
+            double err = fma(s, p.Phi, -v);
+            err = fma(s, p.Plo, err);
+            err = fma(err0, p.Phi, err);
+            out[c] = ldexp(v + err, sh);
+        }
+        if (gi == gj) out[1] = 0.0;
+        pb.C[(int64_t)gj * pb.ldc + gi] = make_double2(out[0], out[1]);
     }
-    if (i == j) out[1] = 0.0;
-    p.C[(int64_t)j * p.ldc + i] = make_double2(out[0], out[1]);
 }
 
-hipError_t launch_crt_final(const OzFinalParams& p, hipStream_t st)
+hipError_t launch_crt_final(const OzFinalParams& p, const TileWork* work, int nwork, hipStream_t st)
 {
-    if (p.n <= 0) return hipSuccess;
-    dim3 grid((p.n + 15) / 16, (p.n + 15) / 16), block(16, 16);
-    hipLaunchKernelGGL(k_crt_final, grid, block, 0, st, p);
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_crt_final, dim3(nwork), dim3(256), 0, st, p, work);
     return hipGetLastError();
 }
 
