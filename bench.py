@@ -302,20 +302,22 @@ def load_traffic(kernel: str, workload: str):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--gpus", type=int, default=1, help="number of GPUs; only 1 is measured")
+    ap.add_argument("--steps", type=int, default=3, help="timed H+S evaluations")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed H+S evaluations before the timed ones")
     ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 4],
                     help="workload; 4 = 64 k-points through the batched entry point")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "int8"])
-    ap.add_argument("--slices", type=int, default=0)
-    ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "int8"],
+                    help="arithmetic of the measured run (int8: split precision)")
+    ap.add_argument("--slices", type=int, default=0, help="split precision: modulus count (0: from the error model)")
+    ap.add_argument("--tile", type=int, default=0, help="tile configuration id (0: default or autotuned)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the self check (a sampled submatrix recomputed as a small k-point)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split-precision comparison run after the FP64 run")
-    ap.add_argument("--split-target", type=float, default=1e-12)
+    ap.add_argument("--split-target", type=float, default=1e-12, help="split precision: relative error target")
     ap.add_argument("--eig", action="store_true",
                     help="also time the generalized eigensolve H x = lambda S x of the result")
     ap.add_argument("--host-out", action="store_true",
