@@ -861,6 +861,9 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, int ng, int flags, 
     // the free memory share one residue pass, which reads the FP64 operands
     // once for the whole group.
     const int64_t ldo = 2 * kd;
+    if ((uint64_t)ldo * (uint64_t)(ic.bm + ic.bn) >= (1ull << 32))
+        return fail(ctx, HSGEN_ERR_UNSUPPORTED, "INT8 contraction depth %lld too large for 32-bit tile offsets",
+                    (long long)kd);
     const size_t arr = (size_t)ldo * gp;
     const int narr = ic.needs_rot == 1 ? 4 : ic.needs_rot == 2 ? 5 : 3;
     const size_t slot_bytes = narr * arr;

@@ -529,15 +529,30 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     const int wn = wave / WGM;
 
     // this lane's source for each of the wave's loads of a stage (stage 0)
-    const int8_t* src[LPW];
+    // this lane's source for each of the wave's loads: a wave load fills 64
+    // chunks = whole columns of one operand (the operands' column counts are
+    // multiples of 64 / CPC), so its base is wave-uniform and each lane adds a
+    // 32-bit offset (the host keeps (BM + BN) columns of an operand under 4 GiB)
+    static_assert((BM * CPC) % 64 == 0 && ((BM + BN) * CPC) % 64 == 0, "a wave load must stay in one operand");
+    const int8_t* base[LPW];
+    uint32_t off[LPW];
 #pragma unroll
     for (int r = 0; r < LPW; ++r) {
-        const int slot = (wave * LPW + r) * 64 + lane;
+        const int g = wave * LPW + r;
+        const int c0 = g * (64 / CPC);            // first column of this wave load (uniform)
+        const int slot = g * 64 + lane;
         const int c = slot / CPC;
         const int q = (slot % CPC) ^ chunk_swz<CPC>(c);
-        if (c < BM) src[r] = pb.Lre + (int64_t)(I0 + c) * pb.ldl + 16 * q;
-        else if (c < BM + BN) src[r] = pb.R + (int64_t)(J0 + c - BM) * pb.ldr + 16 * q;
-        else src[r] = pb.Rrot + (int64_t)(J0 + c - BM - BN) * pb.ldr + 16 * q;
+        if (c0 < BM) {
+            base[r] = pb.Lre + (int64_t)I0 * pb.ldl;
+            off[r] = (uint32_t)(c * pb.ldl + 16 * q);
+        } else if (c0 < BM + BN) {
+            base[r] = pb.R + (int64_t)J0 * pb.ldr;
+            off[r] = (uint32_t)((c - BM) * pb.ldr + 16 * q);
+        } else {
+            base[r] = pb.Rrot + (int64_t)J0 * pb.ldr;
+            off[r] = (uint32_t)((c - BM - BN) * pb.ldr + 16 * q);
+        }
     }
     typedef __attribute__((address_space(3))) char lchar;
     lchar* lbase = (lchar*)ilds;
@@ -545,10 +560,11 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     auto issue = [&](int stage_k, int buf) {
 #pragma unroll
         for (int r = 0; r < LPW; ++r) {
-            const __attribute__((address_space(1))) void* g =
-                (const __attribute__((address_space(1))) void*)(src[r] + (int64_t)stage_k * (2 * KS));
-            __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(lbase + buf * SB +
-                                                                                           (wave * LPW + r) * 1024),
+            const __attribute__((address_space(1))) char* gb =
+                (const __attribute__((address_space(1))) char*)(base[r] + (int64_t)stage_k * (2 * KS));
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(gb + off[r]),
+                                             (__attribute__((address_space(3))) void*)(lbase + buf * SB +
+                                                                                       (wave * LPW + r) * 1024),
                                              16, 0, 0);
         }
     };
@@ -705,7 +721,9 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     X(5, 128, 128, 32, 2, 4)        \
     X(6, 128, 64, 64, 2, 2)
 
-// pipelined kernels: (id, BM, BN, KS, WGM, WGN, NS, ROTR)
+// pipelined kernels: (id, BM, BN, KS, WGM, WGN, NS, ROTR).  Every one of them
+// compiles without register spills: the waits on vmcnt count the operand
+// loads only, and scratch traffic in the loop would break that count.
 #define HSGEN_IGEMM2_CFGS(Y)            \
     Y(11, 256, 128, 64, 4, 2, 3, 0)      \
     Y(12, 256, 128, 32, 4, 2, 4, 0)      \
@@ -718,10 +736,6 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     Y(23, 128, 128, 64, 2, 4, 3, 1)      \
     Y(24, 256, 128, 64, 4, 2, 2, 1)      \
     Y(25, 128, 256, 32, 2, 4, 3, 1)      \
-    Y(26, 128, 256, 64, 2, 2, 3, 0)      \
-    Y(27, 128, 256, 32, 2, 2, 4, 0)      \
-    Y(28, 256, 128, 64, 4, 1, 3, 0)      \
-    Y(29, 256, 256, 32, 4, 2, 3, 0)      \
     Y(31, 128, 256, 64, 2, 2, 3, 2)      \
     Y(32, 128, 256, 32, 2, 2, 4, 2)      \
     Y(33, 256, 128, 64, 4, 1, 3, 2)      \
