@@ -43,7 +43,9 @@ typedef enum {
 typedef enum {
     HSGEN_PREC_FP64 = 0,      /* FP64 MFMA (v_mfma_f64_16x16x4) */
     HSGEN_PREC_OZAKI_INT8 = 1 /* split precision: INT8 slices on INT8 MFMA,
-                                 exact INT32 accumulation, FP64 recombination */
+                                 exact INT32 accumulation, FP64 recombination;
+                                 HSGEN_ERR_UNSUPPORTED if 2 (2K) bytes x 448 columns
+                                 of an operand would exceed 4 GiB (K > 2.3e6) */
 } hsgen_precision;
 
 enum {
@@ -115,7 +117,7 @@ typedef struct {
     double ms_igemm;           /* split precision: GPU time of the INT8 GEMM launches alone
                                   (last modulus set when an automatic retry ran) */
     double err_smp_h;          /* split precision: measured relative Frobenius error of H on
-                                  bn evenly spaced columns (their lower parts) recomputed in FP64 */
+                                  64 evenly spaced columns (their lower parts) recomputed in FP64 */
     double err_smp_s;          /* ... and of S */
 } hsgen_report;
 
