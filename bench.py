@@ -195,8 +195,9 @@ def split_run(args, system, G, H, S, step, fp64_ms, F):
 
 
 def bench_batched(args) -> int:
-    """64 k-points of uneven N_G through hsgen_create_hs_batched (grouped
-    launches, graph captured on the warm-up call and replayed)."""
+    """64 k-points of uneven N_G through hsgen_create_hs_batched (FP64:
+    grouped launches, graph captured on the warm-up call and replayed; split
+    precision: INT8 launches shared by the k-points of a memory batch)."""
     import numpy as np
     import torch
     from hybridhsdla_amd import make_config
@@ -234,7 +235,24 @@ def bench_batched(args) -> int:
     f_contract = float(sum(16.0 * n * n * system.kdim for n in ngs))
     ms_gpu = float(np.mean([r[0]["ms_total"] for r in reps]))
     tflops = F / (ms_step * 1e-3) / 1e12
-    peak = FP64_PEAK if prec == HSGEN_PREC_FP64 else INT8_PEAK
+    if prec == HSGEN_PREC_FP64:
+        roof = {"bound": "mfma", "kernel": "batched graph: k_abgen per k, grouped T application, "
+                "persistent grouped S/H contraction",
+                "achieved": round(f_contract / (ms_gpu * 1e-3) / 1e12, 3), "peak": FP64_PEAK, "unit": "TFLOP/s",
+                "frac": round(f_contract / (ms_gpu * 1e-3) / 1e12 / FP64_PEAK, 4), "traffic": None,
+                "note": "contraction flops over the whole graph's GPU time"}
+    else:
+        # INT8 operations of the residue GEMMs (16 G^2 K per modulus and
+        # k-point) over the GPU time of those launches (each k-point's report
+        # holds its share of the launches it shared)
+        ops = sum(r["slices_used"] * 16.0 * r["ng"] ** 2 * system.kdim for r in reps[-1])
+        ms_ig = float(np.mean([sum(x["ms_igemm"] for x in r) for r in reps]))
+        ach = ops / (ms_ig * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_igemm2_crt (INT8 residue GEMMs, one launch per modulus over "
+                "the tiles of every k-point of a memory batch)",
+                "achieved": round(ach, 1), "peak": INT8_PEAK, "unit": "TOP/s", "frac": round(ach / INT8_PEAK, 4),
+                "traffic": None, "ms_igemm_per_step": round(ms_ig, 2),
+                "moduli": sorted({int(x["slices_used"]) for x in reps[-1]})}
     check = None
     if not args.no_check:
         try:
@@ -273,12 +291,7 @@ def bench_batched(args) -> int:
                    "natoms": system.natoms, "n_k": len(kps), "n_g_total": int(sum(ngs)),
                    "k_dim": system.kdim, "precision": args.precision},
         "flops_per_step": F,
-        "roofline": {"bound": "mfma", "kernel": "batched graph: k_abgen per k, grouped T application, "
-                     "persistent grouped S/H contraction",
-                     "achieved": round(f_contract / (ms_gpu * 1e-3) / 1e12, 3), "peak": peak,
-                     "unit": "TFLOP/s" if prec == HSGEN_PREC_FP64 else "TOP/s",
-                     "frac": round(f_contract / (ms_gpu * 1e-3) / 1e12 / peak, 4), "traffic": None,
-                     "note": "contraction flops over the whole graph's GPU time"},
+        "roofline": roof,
         "self_check": check,
         "per_k_calls": loop,
         "cpu_baseline": cpu,

@@ -160,7 +160,7 @@ struct IgemmLaunchCfg {
 struct OzFinalParams {
     const uint8_t* res;         // residue outputs of modulus 0; modulus m at res + m mod_stride
     int64_t mod_stride;         // bytes
-    OzFinalProblem prob[2];
+    const OzFinalProblem* prob; // [problem id of the INT8 work list], device memory
     int32_t nmod, beta;
     int32_t bm, bn, wgm, wgn;   // INT8 tile: the layout of a slot
     double whi[kMaxModuli];     // u_m / p_m as double-double (u_m = (P / p_m)^-1 mod p_m)

@@ -157,7 +157,12 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sys,
  * generate_matrices (flapw_opt/src/h_and_s.cpp:914-931).
  * H[i], S[i]: kps[i].ng x kps[i].ng with leading dimension ld[i].
  * report may be NULL or point to nk reports (per-k figures are shares of
- * the single batched launch; ms_total is the whole batch in report[0]). */
+ * the single batched launch; ms_total is the whole batch in report[0]).
+ * Split precision: the k-points of a memory batch share their INT8 launches
+ * (one per modulus over the tiles of all k-points that use that many moduli);
+ * each k-point gets the modulus count of its own error model, and its H, S
+ * and report fields equal those of a hsgen_create_hs call, except ms_igemm:
+ * the k-point's share (by ng^2) of the INT8 launches it shared. */
 hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys,
                                      int32_t nk, const hsgen_kpoint* kps,
                                      const hsgen_opts* opts, double* const* H,

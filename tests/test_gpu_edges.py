@@ -7,7 +7,8 @@
   sphbes stops for a maximum lmax < 1, `sphbes.f:29`, which the library
   reports as an argument error);
 - repeated G-vectors (equal rows and columns of H and S);
-- batched k-points of ragged sizes, including one with a single G-vector;
+- batched k-points of ragged sizes, including one with a single G-vector and
+  one with none, in FP64 and split precision;
 - argument errors come back as status codes with a message, never a crash.
 
 Tolerances as in test_gpu_parity.py: relative Frobenius error of the lower
@@ -122,6 +123,19 @@ def test_batched_ragged_sizes_equal_single_calls(ctx, big_list):
     res, _ = ctx.create_HS2_batched(h, kps)
     for (Hb, Sb), k in zip(res, kps):
         H1, S1 = ctx.create_HS2(h, k)
+        assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
+
+
+def test_batched_split_ragged_sizes_equal_single_calls(ctx, big_list):
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = big_list
+    kps = [_subset(kp, [7]), _subset(kp, []), _subset(kp, np.arange(769)), _subset(kp, np.arange(300) * 2),
+           _subset(kp, np.arange(2) + 100)]
+    h = ctx.upload(s)
+    res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    for (Hb, Sb), k, r in zip(res, kps, reps):
+        print(f"ng={k.ng}: {r['slices_used']} moduli", flush=True)
+        H1, S1 = ctx.create_HS2(h, k, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
         assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
 
 

@@ -120,3 +120,40 @@ def test_batched_64_k_points(big_ctx):
         print(f"k {i} (N_G {ngs[i]}): H {eh:.2e} S {es:.2e}", flush=True)
         assert eh <= 1e-13 and es <= 1e-13
     del mats
+
+
+def test_batched_64_k_points_split_precision(big_ctx):
+    """The 64 k-points in split precision through the batched entry point
+    (INT8 launches shared by the k-points of a memory batch): every k-point
+    meets the 1e-12 target by its own error model, three of them against the
+    oracle on a sample, and the largest one equals its single-k call bit for
+    bit."""
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kps = make_config(4)
+    ngs = [kp.ng for kp in kps]
+    h = big_ctx.upload(s)
+    mats = [(torch.zeros((n, n), dtype=torch.complex128, device="cuda"),
+             torch.zeros((n, n), dtype=torch.complex128, device="cuda")) for n in ngs]
+    outs = [(a.data_ptr(), b.data_ptr(), n) for (a, b), n in zip(mats, ngs)]
+    _, reps = big_ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12,
+                                         outputs=outs)
+    torch.cuda.synchronize()
+    nm = sorted({r["slices_used"] for r in reps})
+    print(f"moduli used: {nm}; {reps[0]['ms_total']:.1f} ms", flush=True)
+    for r in reps:
+        assert 2 <= r["slices_used"] <= 20
+        assert r["err_est_h"] <= 1e-12 and r["err_est_s"] <= 1e-12
+        assert r["err_smp_h"] <= 1e-12 and r["err_smp_s"] <= 1e-12
+    for i in (int(np.argmax(ngs)), int(np.argmin(ngs)), len(kps) // 2):
+        eh, es = _sample_parity(s, kps[i], mats[i][0], mats[i][1], seed=i)
+        print(f"k {i} (N_G {ngs[i]}, {reps[i]['slices_used']} moduli): H {eh:.2e} S {es:.2e}", flush=True)
+        assert eh <= 1e-12 and es <= 1e-12
+    i = int(np.argmax(ngs))
+    H1 = torch.zeros_like(mats[i][0])
+    S1 = torch.zeros_like(mats[i][1])
+    r1 = big_ctx.create_hs_device(h, kps[i], H1.data_ptr(), S1.data_ptr(), ngs[i],
+                                  precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    assert r1["slices_used"] == reps[i]["slices_used"]
+    assert torch.equal(H1, mats[i][0]) and torch.equal(S1, mats[i][1])
+    del mats, H1, S1

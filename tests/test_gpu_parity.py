@@ -267,16 +267,47 @@ def test_batched_graph_replay_and_rebuild(ctx):
     assert np.array_equal(np.tril(dev[1][1].cpu().numpy().T), S1)
 
 
-def test_batched_split_precision(ctx):
+@pytest.mark.parametrize("budget", [None, "3000000"])
+def test_batched_split_precision(ctx, monkeypatch, budget):
+    """Split precision over many k-points: the INT8 launches of a batch cover
+    the tiles of all its k-points, each k-point with the modulus count its own
+    error model asks for, so every H and S equals the single-k call's bit for
+    bit (with a small memory cap: over several batches), and the oracle's to
+    the target."""
     from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    if budget:
+        monkeypatch.setenv("HSGEN_BATCH_BYTES", budget)
     s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
-    kps = _kset(s, nk=3, seed=9)
+    kps = _kset(s, nk=4, seed=9)
+    print("batched split ng:", [kp.ng for kp in kps], "budget", budget, flush=True)
     h = ctx.upload(s)
     res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
     for kp, (H, S), r in zip(kps, res, reps):
+        H1, S1, r1 = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+        assert np.array_equal(H, H1) and np.array_equal(S, S1), kp.ng
+        for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
+            assert r[key] == r1[key], (kp.ng, key)
         Hr, Sr = po.reference_hs(s, kp)
         assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
-        assert r["slices_used"] > 0
+        assert r["slices_used"] > 0 and r["ng"] == kp.ng
+    assert reps[0]["ms_total"] > 0
+
+
+def test_batched_split_fixed_moduli_and_flags(ctx):
+    """A fixed modulus count for every k-point, H or S skipped or the upper
+    triangles filled: the same bits as single-k calls (a skipped matrix stays
+    zero in both)."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, _ = tiny_system(natoms=2, lmax=(3, 2), a_cell=7.0)
+    kps = _kset(s, nk=3, seed=4)
+    h = ctx.upload(s)
+    for flags in (0, 1, 2, 4):      # none, HSGEN_FILL_UPPER, HSGEN_SKIP_H, HSGEN_SKIP_S
+        res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, slices=11, flags=flags)
+        for kp, (H, S), r in zip(kps, res, reps):
+            H1, S1 = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=11, flags=flags)
+            assert np.array_equal(H, H1) and np.array_equal(S, S1), (flags, kp.ng)
+            assert not (flags & 2 and H.any()) and not (flags & 4 and S.any())
+            assert r["slices_used"] == 11
 
 
 def test_autotune_picks_and_applies_a_tile(ctx):
