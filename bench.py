@@ -417,7 +417,7 @@ def main(argv=None) -> int:
     }
     abgen_bytes = float(reps[-1]["bytes_abgen"])
     abgen = {
-        "bound": "hbm", "kernel": "k_abgen (A/B generator)",
+        "bound": "hbm", "kernel": "k_abgen_cls (A/B generator, Y_lm and Bessel values shared by rotation / rmt class)",
         "achieved": round(abgen_bytes / (ms_abgen * 1e-3) / 1e9, 1) if ms_abgen > 0 else None,
         "peak": HBM_PEAK, "unit": "GB/s",
         "frac": round(abgen_bytes / (ms_abgen * 1e-3) / 1e9 / HBM_PEAK, 4) if ms_abgen > 0 else None,

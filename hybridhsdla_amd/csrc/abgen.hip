@@ -275,6 +275,247 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
     }
 }
 
+// conj(Y_lm) for l <= lmax at one m >= 0 of the direction (cth, sth, cph,
+// sph), by the ylm4 recurrences (ylm4.f:194-217); yl indexed by lm = l(l+1)+m.
+__device__ inline void ylm_column(int m, int lmax, double cth, double sth, double cph, double sph,
+                                  const double* __restrict__ ynorm, double2* yl)
+{
+    // cos(m phi), sin(m phi) by the Chebyshev recurrence (ylm4.f:208-217)
+    double cm = 1.0, sm = 0.0;
+    if (m >= 1) {
+        double cmm1 = 1.0, smm1 = 0.0;   // m-2 terms
+        cm = cph;
+        sm = sph;
+        const double cph2 = cph + cph;
+        for (int mm = 2; mm <= m; ++mm) {
+            const double cn = cph2 * cm - cmm1;
+            const double sn = cph2 * sm - smm1;
+            cmm1 = cm;
+            smm1 = sm;
+            cm = cn;
+            sm = sn;
+        }
+    }
+    // p(m,m) (ylm4.f:194-206)
+    double fac = 1.0;
+    for (int mm = 0; mm < m; ++mm) {
+        fac = -(mm + mm - 1) * fac;
+        fac = fac * sth;
+    }
+    fac = -(m + m - 1) * fac;
+    double pl2 = 0.0, pl1 = fac;      // p(l-2,m), p(l-1,m)
+    for (int l = m; l <= lmax; ++l) {
+        double pv;
+        if (l == m) pv = fac;
+        else if (l == m + 1) pv = (m + m + 1) * cth * fac;
+        else pv = ((l + l - 1) * cth * pl1 - (l + m - 1) * pl2) / (l - m);
+        if (l > m) {
+            pl2 = pl1;
+            pl1 = pv;
+        }
+        const int lm0 = l * (l + 1);
+        if (m == 0) {
+            yl[lm0] = make_double2(ynorm[lm0] * pv, 0.0);
+        } else {
+            const double yr = pv * cm, yi = pv * sm;
+            const double np = ynorm[lm0 + m], nm = ynorm[lm0 - m];
+            yl[lm0 + m] = make_double2(np * yr, -np * yi);   // conj(Y_{l,m})
+            yl[lm0 - m] = make_double2(yr * nm, yi * nm);    // conj(Y_{l,-m})
+        }
+    }
+}
+
+// l of lm = l(l+1)+m, |m| <= l
+__device__ inline int l_of_lm(int lm)
+{
+    int l = (int)sqrtf((float)lm + 0.5f);
+    if ((l + 1) * (l + 1) <= lm) ++l;
+    if (l * l > lm) --l;
+    return l;
+}
+
+// Class-shared generator: Y_lm depends on the atom only through its
+// rotation, the Bessel values only through its rmt (and the global
+// max_lmax).  Per G column they are computed once per rotation class and
+// once per rmt class; each chunk of atoms then needs only its phases and
+// radial factors before the coalesced stores.  Same arithmetic per element
+// as k_abgen.
+template <bool kCompact>
+__global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
+{
+    extern __shared__ double smem[];
+    const int col = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nl = p.nl;
+    const int L = p.L;
+    const int chunk = p.chunk;
+    double2* s_y = (double2*)smem;                       // [nrot][L]    conj(Y_lm)
+    double* s_bes = (double*)(s_y + p.nrot * L);         // [nrmt][2 nl] fj, dfj
+    double* s_dir = s_bes + p.nrmt * 2 * nl;             // [nrot][4]    cth sth cph sph
+    double* s_rf = s_dir + p.nrot * 4;                   // [chunk][nl][4]
+    double* s_un = s_rf + chunk * nl * 4;                // [chunk][nl]  Zs scale of B rows
+    int* s_off = (int*)(s_un + chunk * nl);              // [chunk+1]    row prefix (compact)
+    int* s_rc = s_off + chunk + 1;                       // [chunk]      rotation class
+
+    if (kCompact && col >= p.ng) {
+        const double2 z = make_double2(0.0, 0.0);
+        double2* zc = p.Z + (int64_t)col * p.ldz;
+        double2* zsc = p.Zs + (int64_t)col * p.ldz;
+        for (int64_t r = tid; r < p.ldz; r += blockDim.x) {
+            zc[r] = z;
+            zsc[r] = z;
+        }
+        return;
+    }
+    const double K0 = p.kpt[0] + p.g[3 * col + 0];
+    const double K1 = p.kpt[1] + p.g[3 * col + 1];
+    const double K2 = p.kpt[2] + p.g[3 * col + 2];
+    const double c0 = p.bmat[0] * K0 + p.bmat[3] * K1 + p.bmat[6] * K2;
+    const double c1 = p.bmat[1] * K0 + p.bmat[4] * K1 + p.bmat[7] * K2;
+    const double c2 = p.bmat[2] * K0 + p.bmat[5] * K1 + p.bmat[8] * K2;
+    const double Knorm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+
+    // per class: Bessel values (threads 0..nrmt-1), directions (64..64+nrot-1)
+    if (tid < p.nrmt) {
+        double* fj = s_bes + tid * 2 * nl;
+        bessel_lds(p.max_lmax, p.atoms[p.rmt_rep[tid]].rmt * Knorm, fj, fj + nl);
+    } else if (tid >= 64 && tid - 64 < p.nrot) {
+        const int c = tid - 64;
+        const double* R = p.atoms[p.rot_rep[c]].rot;
+        double kb0 = R[0] * K0 + R[1] * K1 + R[2] * K2;   // R^T K
+        double kb1 = R[3] * K0 + R[4] * K1 + R[5] * K2;
+        double kb2 = R[6] * K0 + R[7] * K1 + R[8] * K2;
+        const double nrm = sqrt(kb0 * kb0 + kb1 * kb1 + kb2 * kb2);
+        if (nrm != 0.0) {
+            kb0 /= nrm;
+            kb1 /= nrm;
+            kb2 /= nrm;
+        }
+        // ylm4 angles (ylm4.f:170-191)
+        const double small = 1.0e-12;
+        const double xy = kb0 * kb0 + kb1 * kb1;
+        const double r = sqrt(xy + kb2 * kb2);
+        const double rxy = sqrt(xy);
+        double cth, sth, cph, sph;
+        if (r > small) {
+            cth = kb2 / r;
+            sth = rxy / r;
+        } else {
+            sth = 0.0;
+            cth = 1.0;
+        }
+        if (rxy > small) {
+            cph = kb0 / rxy;
+            sph = kb1 / rxy;
+        } else {
+            cph = 1.0;
+            sph = 0.0;
+        }
+        double* dir = s_dir + c * 4;
+        dir[0] = cth;
+        dir[1] = sth;
+        dir[2] = cph;
+        dir[3] = sph;
+    }
+    __syncthreads();
+    // conj(Y_lm) per (rotation class, m >= 0), up to max_lmax
+    for (int q = tid; q < p.nrot * nl; q += blockDim.x) {
+        const int c = q / nl;
+        const int m = q - c * nl;
+        const double* dir = s_dir + c * 4;
+        ylm_column(m, p.max_lmax, dir[0], dir[1], dir[2], dir[3], p.ynorm, s_y + c * L);
+    }
+
+    for (int a0 = 0; a0 < p.natoms; a0 += chunk) {
+        const int na = min(chunk, p.natoms - a0);
+        // radial factors (common * a_l, common * b_l) per (atom, l)
+        for (int q = tid; q < na * nl; q += blockDim.x) {
+            const int t = q / nl;
+            const int l = q - t * nl;
+            const AtomDev& at = p.atoms[a0 + t];
+            if (l == 0) s_rc[t] = at.cls & 0xffff;
+            if (l > at.lmax) continue;
+            const double kp = K0 * at.pos[0] + K1 * at.pos[1] + K2 * at.pos[2];
+            double ps, pc;
+            sincos(2.0 * kPi * kp, &ps, &pc);
+            const double* fj = s_bes + (at.cls >> 16) * 2 * nl;
+            const double* dfj = fj + nl;
+            const RadDev m = p.rad[(int64_t)(a0 + t) * nl + l];
+            const double Au = m.dE_u * Knorm * dfj[l] - m.dE_u_dr * fj[l];
+            const double Bu = -m.u * Knorm * dfj[l] + m.u_dr * fj[l];
+            const double cr = p.vol_prefac * pc * m.inv_wronk;
+            const double ci = p.vol_prefac * ps * m.inv_wronk;
+            double* rf = s_rf + (t * nl + l) * 4;
+            rf[0] = cr * Au;
+            rf[1] = ci * Au;
+            rf[2] = cr * Bu;
+            rf[3] = ci * Bu;
+            s_un[t * nl + l] = m.dE_u_norm;
+        }
+        if (kCompact && tid == 0) {
+            int acc = 0;
+            for (int t = 0; t < na; ++t) {
+                s_off[t] = acc;
+                acc += 2 * p.atoms[a0 + t].nlm;
+            }
+            s_off[na] = acc;
+        }
+        __syncthreads();
+        if (kCompact) {
+            const int nrows = s_off[na];
+            const int64_t base = (int64_t)col * p.ldz + 2 * (int64_t)p.atoms[a0].off;
+            int t = 0;   // the atom of row q; q only grows, so t does
+            for (int q = tid; q < nrows; q += blockDim.x) {
+                while (t + 1 < na && s_off[t + 1] <= q) ++t;
+                const int nlm2 = (s_off[t + 1] - s_off[t]) >> 1;
+                const int r = q - s_off[t];
+                const bool isB = r >= nlm2;
+                const int lm = isB ? r - nlm2 : r;
+                const int l = l_of_lm(lm);
+                const double2 y = s_y[s_rc[t] * L + lm];
+                const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
+                const double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
+                p.Z[base + q] = v;
+                const double sc = isB ? s_un[t * nl + l] : 1.0;
+                p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
+            }
+        } else {
+            const int64_t ld = (int64_t)L * p.natoms;
+            for (int q = tid; q < na * L; q += blockDim.x) {
+                const int t = q / L;
+                const int lm = q - t * L;
+                const int64_t idx = (int64_t)col * ld + (int64_t)(a0 + t) * L + lm;
+                const int l = l_of_lm(lm);
+                if (l <= p.atoms[a0 + t].lmax) {
+                    const double2 y = s_y[s_rc[t] * L + lm];
+                    const double* rf = s_rf + (t * nl + l) * 4;
+                    p.A[idx] = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
+                    p.B[idx] = make_double2(y.x * rf[2] - y.y * rf[3], y.x * rf[3] + y.y * rf[2]);
+                } else {
+                    p.A[idx] = make_double2(0.0, 0.0);
+                    p.B[idx] = make_double2(0.0, 0.0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (kCompact) {
+        const int64_t r0 = 2 * (int64_t)p.kdim;
+        const double2 z = make_double2(0.0, 0.0);
+        for (int64_t r = r0 + tid; r < p.ldz; r += blockDim.x) {
+            p.Z[(int64_t)col * p.ldz + r] = z;
+            p.Zs[(int64_t)col * p.ldz + r] = z;
+        }
+    }
+}
+
+size_t abgen_cls_lds_bytes(int chunk, int nl, int L, int nrot, int nrmt)
+{
+    return 16 * (size_t)nrot * L +
+           sizeof(double) * ((size_t)nrmt * 2 * nl + (size_t)nrot * 4 + (size_t)chunk * nl * 5) +
+           sizeof(int) * (size_t)(2 * chunk + 1);
+}
+
 size_t abgen_lds_bytes(int chunk, int nl, int L)
 {
     return sizeof(double) * ((size_t)chunk * nl * 4 + (size_t)chunk * 4 + (size_t)chunk * 2 * L +
@@ -284,8 +525,16 @@ size_t abgen_lds_bytes(int chunk, int nl, int L)
 
 hipError_t launch_abgen(const AbgenParams& p, bool compact, hipStream_t st)
 {
-    size_t lds = abgen_lds_bytes(p.chunk, p.nl, p.L);
     dim3 grid(p.ncols), block(256);
+    if (p.nrot > 0) {
+        const size_t lds = abgen_cls_lds_bytes(p.chunk, p.nl, p.L, p.nrot, p.nrmt);
+        if (compact)
+            hipLaunchKernelGGL(k_abgen_cls<true>, grid, block, lds, st, p);
+        else
+            hipLaunchKernelGGL(k_abgen_cls<false>, grid, block, lds, st, p);
+        return hipGetLastError();
+    }
+    size_t lds = abgen_lds_bytes(p.chunk, p.nl, p.L);
     if (compact)
         hipLaunchKernelGGL(k_abgen<true>, grid, block, lds, st, p);
     else

@@ -91,6 +91,8 @@ struct hsgen_sys {
     // geometry
     std::vector<std::vector<std::complex<double>>> h_taa, h_tab, h_tbb;
     std::vector<int> h_owner;          // per atom: the atom whose copies hold its T
+    // first atom of each rotation class and of each rmt class (AtomDev::cls)
+    std::vector<int32_t> rot_rep, rmt_rep;
 };
 
 namespace {
@@ -480,10 +482,17 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
         at.lmax = lm;
         at.nlm = (lm + 1) * (lm + 1);
         at.off = off;
-        at.pad = 0;
         at.rmt = d->rmt[a];
         for (int i = 0; i < 3; ++i) at.pos[i] = d->pos[3 * a + i];
         for (int i = 0; i < 9; ++i) at.rot[i] = d->rotmat[9 * a + i];
+        // classes of bitwise-equal rotations and radii (they share Y_lm and
+        // the Bessel values in the generator)
+        int rc = 0, mc = 0;
+        while (rc < (int)s->rot_rep.size() && std::memcmp(s->atoms[s->rot_rep[rc]].rot, at.rot, sizeof at.rot)) ++rc;
+        if (rc == (int)s->rot_rep.size()) s->rot_rep.push_back(a);
+        while (mc < (int)s->rmt_rep.size() && std::memcmp(&s->atoms[s->rmt_rep[mc]].rmt, &at.rmt, sizeof at.rmt)) ++mc;
+        if (mc == (int)s->rmt_rep.size()) s->rmt_rep.push_back(a);
+        at.cls = (rc & 0xffff) | ((mc & 0xffff) << 16);
         off += at.nlm;
         s->sum_nlm2 += (int64_t)at.nlm * at.nlm;
         for (int l = 0; l < s->nl; ++l) {
@@ -605,10 +614,25 @@ AbgenParams abgen_params(const hsgen_sys* s, const hsgen_kpoint* kp, const int32
     p.L = s->L;
     p.ng = kp->ng;
     p.kdim = s->kdim;
-    // atoms per LDS chunk: keep the block's LDS at <= 48 KiB
+    // atoms per LDS chunk: keep the block's LDS at <= 48 KiB.  With few
+    // rotation and rmt classes the class-shared generator runs (it needs no
+    // per-atom Y_lm in LDS, so its chunks are larger).
+    const int nrot = (int)s->rot_rep.size(), nrmt = (int)s->rmt_rep.size();
+    if (nrot <= kAbgenMaxCls && nrmt <= kAbgenMaxCls &&
+        abgen_cls_lds_bytes(1, s->nl, s->L, nrot, nrmt) <= 48 * 1024) {
+        int chunk = 32;
+        while (chunk > 1 && abgen_cls_lds_bytes(chunk, s->nl, s->L, nrot, nrmt) > 48 * 1024) chunk /= 2;
+        p.chunk = chunk;
+        p.nrot = nrot;
+        p.nrmt = nrmt;
+        for (int c = 0; c < nrot; ++c) p.rot_rep[c] = s->rot_rep[c];
+        for (int c = 0; c < nrmt; ++c) p.rmt_rep[c] = s->rmt_rep[c];
+        return p;
+    }
     int chunk = 16;
     while (chunk > 1 && abgen_lds_bytes(chunk, s->nl, s->L) > 48 * 1024) chunk /= 2;
     p.chunk = chunk;
+    p.nrot = 0;   // general generator
     return p;
 }
 

@@ -7,6 +7,7 @@
 namespace hsgen {
 
 constexpr int kMaxLmax = 24;            // largest supported l cutoff
+constexpr int kAbgenMaxCls = 16;        // rotation / rmt classes of the class-shared generator
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
 // Per-atom data of the A/B generator (AtomData fields, AtomData.h:16-52).
@@ -14,7 +15,8 @@ struct AtomDev {
     int32_t lmax;      // spherical cutoff of this atom
     int32_t nlm;       // (lmax+1)^2
     int32_t off;       // sum of nlm over preceding atoms (row offset / 2 in Z)
-    int32_t pad;
+    int32_t cls;       // rotation class (bits 0-15) and rmt class (bits 16-31):
+                       // atoms of one class have bitwise-equal rot / rmt
     double rmt;        // muffin-tin radius
     double pos[3];     // fractional position
     double rot[9];     // column-major R_atom * bmat
@@ -61,6 +63,11 @@ struct AbgenParams {
     int32_t ng, ncols;        // columns written (compact: padded; ref: ng)
     int32_t chunk;            // atoms per LDS chunk
     int32_t kdim;             // K = sum nlm (compact)
+    // class-shared generator (nrot > 0): Y_lm once per rotation class and the
+    // Bessel values once per rmt class, from the classes' first atoms
+    int32_t nrot, nrmt;
+    int32_t rot_rep[kAbgenMaxCls];
+    int32_t rmt_rep[kAbgenMaxCls];
     // compact
     double2* Z;
     double2* Zs;
@@ -87,6 +94,7 @@ struct PackParams {
 
 hipError_t launch_abgen(const AbgenParams& p, bool compact, hipStream_t st);
 size_t abgen_lds_bytes(int chunk, int nl, int L);
+size_t abgen_cls_lds_bytes(int chunk, int nl, int L, int nrot, int nrmt);
 hipError_t launch_pack_ref(const PackParams& p, int ncols, hipStream_t st);
 
 struct ZgemmLaunchCfg {

@@ -3,6 +3,7 @@
 - no G-vectors, one G-vector;
 - sizes on either side of every tile and padding boundary (128, 256, 768),
   in FP64 and split precision;
+- generator atom classes: many distinct rotations / muffin-tin radii;
 - atoms with lmax = 0 (the reference allows any lmax_a <= max_lmax; Fleur's
   sphbes stops for a maximum lmax < 1, `sphbes.f:29`, which the library
   reports as an argument error);
@@ -78,6 +79,27 @@ def test_ragged_sizes_split_precision(ctx, big_list, ng):
 def test_lmax_zero_atoms(ctx):
     s, kp = tiny_system(natoms=4, lmax=(0, 3, 1), ng_target=60, seed=11)
     assert 0 in s.lmax
+    h = ctx.upload(s)
+    A, B = ctx.construct_AB(h, kp)
+    Ar, Br = po.construct_ab(s, kp)
+    assert np.linalg.norm(A - Ar) <= TOL * np.linalg.norm(Ar)
+    assert np.linalg.norm(B - Br) <= TOL * np.linalg.norm(Br)
+    H, S = ctx.create_HS2(h, kp)
+    Hr, Sr = po.reference_hs(s, kp)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+
+
+@pytest.mark.parametrize("natoms,rotate,own_rmt", [(16, True, True), (18, True, False), (17, False, True),
+                                                   (40, False, False)])
+def test_generator_rotation_and_rmt_classes(ctx, natoms, rotate, own_rmt):
+    """Atoms with bitwise-equal rotations (radii) share Y_lm (Bessel values)
+    in the generator; up to 16 classes of each take the class-shared kernel,
+    more the general one (here 16/16, 18 rotations, 17 radii, and 40 atoms
+    of one class in two chunks).  Both against the oracle."""
+    s, kp = tiny_system(natoms=natoms, lmax=(2, 3, 1), ng_target=40, seed=13, rotate=rotate)
+    if own_rmt:
+        s.rmt = np.linspace(1.2, 1.9, natoms)
+    print(f"natoms={natoms} rotate={rotate} own_rmt={own_rmt}", flush=True)
     h = ctx.upload(s)
     A, B = ctx.construct_AB(h, kp)
     Ar, Br = po.construct_ab(s, kp)
