@@ -184,34 +184,47 @@ __device__ inline int sym_mod(double x, double p, double invp, int lo, int hi)
     return r;
 }
 
+// One lane per 4 consecutive rows of a column: 4 complex values in, per
+// modulus one 8-byte store of their 4 residue pairs (rows is a multiple of 4).
 __global__ __launch_bounds__(256) void k_residues(OzResParams p)
 {
     const int col = blockIdx.y;
-    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (r0 >= p.rows) return;
     const int sh = p.beta - p.e[col];
-    const double2 v = r0 < p.rows_src ? p.X[(int64_t)col * p.ldx + r0] : make_double2(0.0, 0.0);
-    const double xr = rint(ldexp(v.x, sh));
-    const double xi = rint(ldexp(v.y, sh));
+    const double2* xc = p.X + (int64_t)col * p.ldx;
+    double xr[4], xi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double2 v = r0 + k < p.rows_src ? xc[r0 + k] : make_double2(0.0, 0.0);
+        xr[k] = rint(ldexp(v.x, sh));
+        xi[k] = rint(ldexp(v.y, sh));
+    }
     for (int m = 0; m < p.nmod; ++m) {
         const double pm = (double)p.p[m];
         const int hi = (p.p[m] - 1) / 2;
         const int lo = -(p.p[m] / 2);
-        const int rr = sym_mod(xr, pm, p.invp[m], lo, hi);
-        const int ri = sym_mod(xi, pm, p.invp[m], lo, hi);
-        char2* o = (char2*)(p.out[m] + (int64_t)col * p.ldo) + r0;
-        *o = make_char2((signed char)rr, (signed char)ri);
-        if (p.out_rot[m]) {
-            char2* orot = (char2*)(p.out_rot[m] + (int64_t)col * p.ldo) + r0;
-            *orot = p.rot_kind == 1 ? make_char2((signed char)(-ri), (signed char)rr)
-                                    : make_char2((signed char)ri, (signed char)(-rr));
+        uint32_t w[2] = {0u, 0u}, wr[2] = {0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int rr = sym_mod(xr[k], pm, p.invp[m], lo, hi);
+            const int ri = sym_mod(xi[k], pm, p.invp[m], lo, hi);
+            // bytes (re, im) of row r0 + k at byte offset 2 k
+            const int s = 16 * (k & 1);
+            w[k >> 1] |= ((uint32_t)(rr & 0xff) | ((uint32_t)(ri & 0xff) << 8)) << s;
+            const int a = p.rot_kind == 1 ? -ri : ri;
+            const int b = p.rot_kind == 1 ? rr : -rr;
+            wr[k >> 1] |= ((uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8)) << s;
         }
+        *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(w[0], w[1]);
+        if (p.out_rot[m]) *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(wr[0], wr[1]);
     }
 }
 
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st)
 {
-    dim3 grid((p.rows + 255) / 256, ncols), block(256);
+    if (p.rows % 4 != 0 || p.ldo % 8 != 0) return hipErrorInvalidValue;
+    dim3 grid((p.rows / 4 + 255) / 256, ncols), block(256);
     hipLaunchKernelGGL(k_residues, grid, block, 0, st, p);
     return hipGetLastError();
 }
