@@ -636,11 +636,25 @@ AbgenParams abgen_params(const hsgen_sys* s, const hsgen_kpoint* kp, const int32
     return p;
 }
 
-// W_a = M_a Z_a for all atoms (one grouped launch).
-void add_tmul_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& work,
-                       const hsgen_sys* s, const Geometry& geo, int ng, const double2* Z, double2* W)
+// Tile configuration of the T application: HSGEN_TAPP_TILE (tuning) if it
+// names a configuration whose tiles divide the contraction's (M is padded
+// for those), else the contraction's own.
+int tapp_cfg(const Geometry& geo)
 {
-    const int ntn = (int)((ng + geo.t.bn - 1) / geo.t.bn);
+    if (const char* e = std::getenv("HSGEN_TAPP_TILE")) {
+        const int id = std::atoi(e);
+        const ZgemmLaunchCfg c = zgemm_cfg(id);
+        if (c.bm > 0 && geo.t.bk % c.bk == 0 && geo.t.bm % c.bm == 0) return id;
+    }
+    return geo.cfg;
+}
+
+// W_a = M_a Z_a for all atoms over ncols columns (one grouped launch), in
+// tiles of the T-application configuration tc.
+void add_tmul_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& work, const hsgen_sys* s,
+                       const Geometry& geo, const ZgemmLaunchCfg& tc, int64_t ncols, const double2* Z, double2* W)
+{
+    const int ntn = (int)((ncols + tc.bn - 1) / tc.bn);
     for (int a = 0; a < s->natoms; ++a) {
         const AtomDev& at = s->atoms[a];
         GemmProblem p;
@@ -651,12 +665,12 @@ void add_tmul_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& w
         p.C = W + 2 * (int64_t)at.off;
         p.ldc = geo.ldz;
         p.m = 2 * at.nlm;
-        p.n = ng;
+        p.n = (int32_t)ncols;
         p.kdepth = s->m_ld[a];
         p.lower = 0;
         const int id = (int)probs.size();
         probs.push_back(p);
-        const int ntm = (p.m + geo.t.bm - 1) / geo.t.bm;
+        const int ntm = (p.m + tc.bm - 1) / tc.bm;
         for (int bi = 0; bi < ntm; ++bi)
             for (int bj = 0; bj < ntn; ++bj) work.push_back(TileWork{id, bi, bj, 0});
     }
@@ -1376,8 +1390,9 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
     std::vector<TileWork> work;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     if (need_h) {
-        add_tmul_problems(probs, work, s, geo, ng, Z, W);
-        st = run_grouped(ctx, geo.cfg, probs, work);
+        const int tc = tapp_cfg(geo);
+        add_tmul_problems(probs, work, s, geo, zgemm_cfg(tc), ng, Z, W);
+        st = run_grouped(ctx, tc, probs, work);
         if (st) return st;
         // rows 2K .. ldz-1 of W are not written by the T application; clear
         // them so that no earlier, larger system's values remain there (the
@@ -1474,6 +1489,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
     const Geometry geo = geometry(o.tile, s->kdim, max_ng);
     const int64_t tm = std::max(geo.t.bm, geo.t.bn);
+    const int tc = tapp_cfg(geo);
     hsgen_status st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
     if (st) return st;
     // largest k first
@@ -1554,7 +1570,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             ldS[q] = ts ? kps[i].ng : ld[i];
         }
         // call signature
-        std::vector<int64_t> sig{(int64_t)s->serial, (int64_t)flags, geo.cfg, (int64_t)(intptr_t)Z,
+        std::vector<int64_t> sig{(int64_t)s->serial, (int64_t)flags, geo.cfg, tc, (int64_t)(intptr_t)Z,
                                  (int64_t)(intptr_t)Zs, (int64_t)(intptr_t)W, (int64_t)(intptr_t)s->d_M,
                                  (int64_t)(intptr_t)ctx->btmp.p, (int64_t)ks.size()};
         for (size_t q = 0; q < ks.size(); ++q) {
@@ -1584,28 +1600,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             // problems: T application over all columns, then S and H per k
             std::vector<GemmProblem> probs;
             std::vector<TileWork> tw_t, tw_c;
-            if (do_h) {
-                const int ntn = (int)((ncols + geo.t.bn - 1) / geo.t.bn);
-                for (int a = 0; a < s->natoms; ++a) {
-                    const AtomDev& at = s->atoms[a];
-                    GemmProblem p;
-                    p.L = s->d_M + s->m_off[a];
-                    p.ldl = s->m_ld[a];
-                    p.R = Z + 2 * (int64_t)at.off;
-                    p.ldr = geo.ldz;
-                    p.C = W + 2 * (int64_t)at.off;
-                    p.ldc = geo.ldz;
-                    p.m = 2 * at.nlm;
-                    p.n = (int32_t)ncols;
-                    p.kdepth = s->m_ld[a];
-                    p.lower = 0;
-                    const int id = (int)probs.size();
-                    probs.push_back(p);
-                    const int ntm = (p.m + geo.t.bm - 1) / geo.t.bm;
-                    for (int bi = 0; bi < ntm; ++bi)
-                        for (int bj = 0; bj < ntn; ++bj) tw_t.push_back(TileWork{id, bi, bj, 0});
-                }
-            }
+            if (do_h) add_tmul_problems(probs, tw_t, s, geo, zgemm_cfg(tc), ncols, Z, W);
             for (size_t q = 0; q < ks.size(); ++q) {
                 const int i = ks[q];
                 const size_t p0 = probs.size();
@@ -1650,7 +1645,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             const GemmProblem* dp = (const GemmProblem*)bg.probs.p;
             const TileWork* dw = (const TileWork*)bg.work.p;
             if (ce == hipSuccess && do_h) {
-                ce = launch_zgemm(geo.cfg, dp, dw, (int)tw_t.size(), ctx->stream, pgrid);
+                ce = launch_zgemm(tc, dp, dw, (int)tw_t.size(), ctx->stream, tc == geo.cfg ? pgrid : 0);
                 if (ce == hipSuccess)
                     ce = launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)ncols, ctx->stream);
             }
@@ -1876,27 +1871,9 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
             // T application over every column of the batch in one grouped launch
             std::vector<GemmProblem> probs;
             std::vector<TileWork> work;
-            const int ntn = (int)((ncols + geo.t.bn - 1) / geo.t.bn);
-            for (int a = 0; a < s->natoms; ++a) {
-                const AtomDev& at = s->atoms[a];
-                GemmProblem p;
-                p.L = s->d_M + s->m_off[a];
-                p.ldl = s->m_ld[a];
-                p.R = Z + 2 * (int64_t)at.off;
-                p.ldr = geo.ldz;
-                p.C = W + 2 * (int64_t)at.off;
-                p.ldc = geo.ldz;
-                p.m = 2 * at.nlm;
-                p.n = (int32_t)ncols;
-                p.kdepth = s->m_ld[a];
-                p.lower = 0;
-                const int id = (int)probs.size();
-                probs.push_back(p);
-                const int ntm = (p.m + geo.t.bm - 1) / geo.t.bm;
-                for (int bi = 0; bi < ntm; ++bi)
-                    for (int bj = 0; bj < ntn; ++bj) work.push_back(TileWork{id, bi, bj, 0});
-            }
-            st = run_grouped(ctx, geo.cfg, probs, work);
+            const int tc = tapp_cfg(geo);
+            add_tmul_problems(probs, work, s, geo, zgemm_cfg(tc), ncols, Z, W);
+            st = run_grouped(ctx, tc, probs, work);
             if (st) return st;
             HIPCHK(launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)ncols, ctx->stream));
         }
