@@ -1,5 +1,6 @@
 """The benchmark workloads at their full sizes on the GPU, against the oracle.
 
+- 32 atoms / 4 types, lmax 10, N_G = 6002, K = 3872 (whole matrices against the oracle);
 - 256 atoms / 8 types, lmax 12, N_G = 24995, K = 43264 (one k-point);
 - 48 atoms / 3 types, lmax 8/10/12, 64 k-points with N_G 4129..8346 (batched).
 
@@ -72,6 +73,30 @@ def cfg2():
     from hybridhsdla_amd import make_config
     s, kps = make_config(2)
     return s, kps[0]
+
+
+def test_medium_workload_whole_matrices(big_ctx):
+    """32 atoms / 4 types, lmax 10, N_G = 6002, K = 3872: the oracle forms the
+    whole H and S here (about 20 s of CPU BLAS), so FP64 and split precision
+    are compared on every lower-triangle element."""
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    from oracle import pyoracle as po
+    from tests.helpers import rel_fro_lower
+    s, kps = make_config(1)
+    kp = kps[0]
+    print(f"32 atoms, N_G={kp.ng}, K={s.kdim}", flush=True)
+    h = big_ctx.upload(s)
+    H, S = big_ctx.create_HS2(h, kp)
+    Hs, Ss, rep = big_ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+    Hr, Sr = po.reference_hs(s, kp)
+    eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
+    fh, fs = rel_fro_lower(Hs, Hr), rel_fro_lower(Ss, Sr)
+    print(f"FP64 vs oracle: H {eh:.2e} S {es:.2e}; split ({rep['slices_used']} moduli): H {fh:.2e} S {fs:.2e}, "
+          f"model {rep['err_est_h']:.2e} {rep['err_est_s']:.2e}", flush=True)
+    assert eh <= 1e-13 and es <= 1e-13
+    assert fh <= 1e-12 and fs <= 1e-12
+    assert not np.any(np.triu(H, 1)) and not np.any(np.triu(Hs, 1))
 
 
 def test_largest_k_point_fp64_and_split(big_ctx, cfg2):
