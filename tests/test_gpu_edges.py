@@ -4,6 +4,7 @@
 - sizes on either side of every tile and padding boundary (128, 256, 768),
   in FP64 and split precision;
 - generator atom classes: many distinct rotations / muffin-tin radii;
+- an l cutoff of 20 next to 7;
 - atoms with lmax = 0 (the reference allows any lmax_a <= max_lmax; Fleur's
   sphbes stops for a maximum lmax < 1, `sphbes.f:29`, which the library
   reports as an argument error);
@@ -108,6 +109,26 @@ def test_generator_rotation_and_rmt_classes(ctx, natoms, rotate, own_rmt):
     H, S = ctx.create_HS2(h, kp)
     Hr, Sr = po.reference_hs(s, kp)
     assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_high_lmax(ctx, rotate):
+    """An l cutoff of 20 (L = 441, T blocks of 882 rows) next to lmax 7:
+    high-l Bessel and Y_lm recurrences, smaller generator chunks, T-application
+    row blocks of 4 tiles; FP64 and split precision against the oracle."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=2, lmax=(20, 7), ng_target=80, seed=17, rotate=rotate)
+    print(f"lmax {list(s.lmax)} ng={kp.ng} rotate={rotate}", flush=True)
+    h = ctx.upload(s)
+    A, B = ctx.construct_AB(h, kp)
+    Ar, Br = po.construct_ab(s, kp)
+    assert np.linalg.norm(A - Ar) <= TOL * np.linalg.norm(Ar)
+    assert np.linalg.norm(B - Br) <= TOL * np.linalg.norm(Br)
+    H, S = ctx.create_HS2(h, kp)
+    Hr, Sr = po.reference_hs(s, kp)
+    assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+    H, S = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
 
 
 def test_max_lmax_zero_is_an_argument_error(ctx):
