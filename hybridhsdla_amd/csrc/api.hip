@@ -928,7 +928,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
             fr += ctx->oz_res.bytes;   // the current residue buffer may be regrown
             group = (int)std::max<size_t>(1, std::min<size_t>((size_t)nmod, (size_t)(0.6 * fr) / slot_bytes));
         }
-        group = std::min(group, 8);
+        group = std::min(group, 16);
         // the same number of passes with equal groups (less memory)
         const int npass = (nmod + group - 1) / group;
         group = (nmod + npass - 1) / npass;
@@ -1746,9 +1746,10 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
     // device copies of host outputs, the stored residue outputs (2 bytes per
     // element of every S/H tile and modulus, counted for 14 moduli or the
     // requested count) and the residue arrays of as many moduli as one
-    // residue pass converts when the moduli take two passes of at most 8
-    // (each pass reads Z, Zs, W once: batches sized for more passes would
-    // read them more often)
+    // residue pass converts when the moduli take two passes of at most 16
+    // (HSGEN_RESIDUE_PASSES for tuning; each pass reads Z, Zs, W once, and
+    // one pass per batch saves those reads but needs more, smaller batches,
+    // which cost as much again at 64 k-points)
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     size_t held = 0;
@@ -1758,7 +1759,10 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
     if (const char* e = std::getenv("HSGEN_BATCH_BYTES"))   // cap (tuning and tests)
         if (std::strtoull(e, nullptr, 10) > 0) budget = std::min<size_t>(budget, std::strtoull(e, nullptr, 10));
     const int nmod_est = o.slices > 0 ? o.slices : 14;
-    const int npass = std::max(2, (nmod_est + 7) / 8);
+    int npass = 2;
+    if (const char* e = std::getenv("HSGEN_RESIDUE_PASSES"))   // tuning
+        npass = std::max(1, std::atoi(e));
+    npass = std::max(npass, (nmod_est + 15) / 16);
     const int gslots = (nmod_est + npass - 1) / npass;
     auto ncols_of = [&](int i) { return roundup(std::max(kps[i].ng, 1), tm); };
     std::vector<size_t> need(nk, 0);
