@@ -33,9 +33,10 @@
 
 using namespace hsgen;
 
-static const int kDefaultTile = 8;
+static const int kDefaultTile = 11;
+static const int kFallbackTile = 8;   // register-staged, 64-bit addressing
 static const int kIgemmCfg = 51;     // default INT8 kernel configuration (256 x 192 tiles)
-static const int kSampleTile = 3;    // FP64 tile of the split-precision sample: 64 x 64, so 64 columns
+static const int kSampleTile = 12;   // FP64 tile of the split-precision sample: 64 x 64, so 64 columns
 
 struct DevBuf {
     void* p = nullptr;
@@ -204,9 +205,16 @@ Geometry geometry(int tile, int kdim, int ng, int64_t col_align = 1)
         g.cfg = kDefaultTile;
         g.t = zgemm_cfg(g.cfg);
     }
-    const int64_t tm = std::max<int64_t>({(int64_t)g.t.bm, (int64_t)g.t.bn, col_align});
     g.kpad = roundup(2 * (int64_t)kdim, g.t.bk);
     g.ldz = g.kpad + g.t.bk;
+    if (zgemm_offs32(g.cfg) && (int64_t)std::max(g.t.bm, g.t.bn) * g.ldz * 16 >= (int64_t(1) << 32)) {
+        // operand panel beyond 32-bit offsets: the register-staged kernel
+        g.cfg = kFallbackTile;
+        g.t = zgemm_cfg(g.cfg);
+        g.kpad = roundup(2 * (int64_t)kdim, g.t.bk);
+        g.ldz = g.kpad + g.t.bk;
+    }
+    const int64_t tm = std::max<int64_t>({(int64_t)g.t.bm, (int64_t)g.t.bn, col_align});
     g.gp = roundup(std::max(ng, 1), tm);
     return g;
 }
@@ -644,7 +652,9 @@ int tapp_cfg(const Geometry& geo)
     if (const char* e = std::getenv("HSGEN_TAPP_TILE")) {
         const int id = std::atoi(e);
         const ZgemmLaunchCfg c = zgemm_cfg(id);
-        if (c.bm > 0 && geo.t.bk % c.bk == 0 && geo.t.bm % c.bm == 0) return id;
+        if (c.bm > 0 && geo.t.bk % c.bk == 0 && geo.t.bm % c.bm == 0 &&
+            !(zgemm_offs32(id) && (int64_t)std::max(c.bm, c.bn) * geo.ldz * 16 >= (int64_t(1) << 32)))
+            return id;
     }
     return geo.cfg;
 }

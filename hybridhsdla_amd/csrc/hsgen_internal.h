@@ -101,6 +101,7 @@ struct ZgemmLaunchCfg {
     int bm, bn, bk, threads;
 };
 ZgemmLaunchCfg zgemm_cfg(int id);
+bool zgemm_offs32(int id);   // operand panels addressed with 32-bit offsets: BM or BN columns x ld < 4 GiB
 hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
                         hipStream_t st, int max_grid = 0);
 hipError_t launch_clear_rows(double2* X, int64_t ld, int64_t r0, int64_t r1, int ncols, hipStream_t st);
