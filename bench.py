@@ -403,10 +403,14 @@ def main(argv=None) -> int:
     contract_tf = f_contract / (ms_contract * 1e-3) / 1e12
     peak = FP64_PEAK if prec == HSGEN_PREC_FP64 else INT8_PEAK
     workload_tag = f"cfg{args.config}"
-    traffic = load_traffic("zgemm_lh_contract", workload_tag)
+    tile_used = int(reps[-1]["tile_used"])
+    # tiles 9..12 are the pipelined LDS-DMA kernel (k_zgemm2_lh), 1..8 the
+    # register-staged one (k_zgemm_lh)
+    kname = "zgemm2_lh" if tile_used >= 9 else "zgemm_lh"
+    traffic = load_traffic(f"{kname}_contract", workload_tag)
     roofline = {
         "bound": "mfma",
-        "kernel": "k_zgemm_lh (S and H lower-triangle tiles, one grouped launch)",
+        "kernel": f"k_{kname} tile {tile_used} (S and H lower-triangle tiles, one grouped launch)",
         "achieved": round(contract_tf, 3),
         "peak": peak,
         "unit": "TFLOP/s",
@@ -415,6 +419,17 @@ def main(argv=None) -> int:
         "algorithmic_flops_per_launch": f_contract,
         "avg_launch_ms": round(ms_contract, 3),
     }
+    if prec != HSGEN_PREC_FP64:
+        # split precision as the main mode: the INT8 residue GEMMs dominate
+        nmod = int(reps[-1]["slices_used"])
+        ms_ig = float(np.mean([r["ms_igemm"] for r in reps]))
+        ach = nmod * f_contract / (ms_ig * 1e-3) / 1e12
+        roofline = {
+            "bound": "mfma", "kernel": f"k_igemm2_crt tile {tile_used} (INT8 residue GEMMs, one launch per modulus)",
+            "achieved": round(ach, 1), "peak": INT8_PEAK, "unit": "TOP/s", "frac": round(ach / INT8_PEAK, 4),
+            "traffic": (load_traffic("igemm2_crt", workload_tag) or {}).get("bytes_per_launch"),
+            "algorithmic_ops_per_launch": f_contract, "avg_launch_ms": round(ms_ig / nmod, 3),
+        }
     abgen_bytes = float(reps[-1]["bytes_abgen"])
     abgen = {
         "bound": "hbm", "kernel": "k_abgen_cls (A/B generator, Y_lm and Bessel values shared by rotation / rmt class)",
