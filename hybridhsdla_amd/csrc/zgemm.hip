@@ -313,37 +313,39 @@ __device__ __forceinline__ void zgemm2_tile(const GemmProblem* __restrict__ prob
 
     const int nst = pb.kdepth / BK;
     constexpr int NKS = BK / 4;
-    // stage 0 in, published; stage 1 in flight
-    issue(0, 0);
-    __builtin_amdgcn_s_waitcnt(kVmcnt0);
-    asm volatile("s_barrier" ::: "memory");
-    issue(min(1, nst - 1), 1);
-    v2d fa[RB], fb[CB];
-    read_frags(0, 0, fa, fb);
-    for (int s = 0; s < nst; ++s) {
+    if (nst > 0) {
+        // stage 0 in, published; stage 1 in flight
+        issue(0, 0);
+        __builtin_amdgcn_s_waitcnt(kVmcnt0);
+        asm volatile("s_barrier" ::: "memory");
+        issue(min(1, nst - 1), 1);
+        v2d fa[RB], fb[CB];
+        read_frags(0, 0, fa, fb);
+        for (int s = 0; s < nst; ++s) {
 #pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-            v2d a[RB], b[CB];
+            for (int ks = 0; ks < NKS; ++ks) {
+                v2d a[RB], b[CB];
 #pragma unroll
-            for (int i = 0; i < RB; ++i) a[i] = fa[i];
+                for (int i = 0; i < RB; ++i) a[i] = fa[i];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) b[j] = fb[j];
-            if (ks + 1 < NKS) {
-                read_frags(s & 1, ks + 1, fa, fb);
-            } else if (s + 1 < nst) {
-                // stage s+1 landed (this wave's loads; the barrier publishes
-                // everyone's) and this wave's reads of stage s are complete;
-                // after the barrier no wave reads stage s, whose buffer takes
-                // stage s+2 (the last stage again at the end: in bounds, unused)
-                __builtin_amdgcn_s_waitcnt(kVmcnt0Lgkm0);
-                asm volatile("s_barrier" ::: "memory");
-                issue(min(s + 2, nst - 1), s & 1);
-                read_frags((s + 1) & 1, 0, fa, fb);
+                for (int j = 0; j < CB; ++j) b[j] = fb[j];
+                if (ks + 1 < NKS) {
+                    read_frags(s & 1, ks + 1, fa, fb);
+                } else if (s + 1 < nst) {
+                    // stage s+1 landed (this wave's loads; the barrier publishes
+                    // everyone's) and this wave's reads of stage s are complete;
+                    // after the barrier no wave reads stage s, whose buffer takes
+                    // stage s+2 (the last stage again at the end: in bounds, unused)
+                    __builtin_amdgcn_s_waitcnt(kVmcnt0Lgkm0);
+                    asm volatile("s_barrier" ::: "memory");
+                    issue(min(s + 2, nst - 1), s & 1);
+                    read_frags((s + 1) & 1, 0, fa, fb);
+                }
+                mfmas(a, b);
             }
-            mfmas(a, b);
         }
-    }
-    __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_waitcnt(0);
+    }   // a zero-depth problem stores zeros
 
     // epilogue: lane holds D[(lane>>4) + 4 q][lane & 15] in register q
     const bool lower = pb.lower != 0;
