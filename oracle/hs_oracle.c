@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the reference
  * algorithm of the HybridHSDLA hot path so that the HIP implementation can be
  * checked against it.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it, and only as the checker or the
+ * cpu_baseline leg of bench.py may load it, and only to compare results with or as the
  * timed CPU baseline - never as the product path.
  *
  * Restated functions and the reference lines they follow:

@@ -1,8 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/hs_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
-product path (hybridhsdla_amd) never imports it.
+cpu_baseline leg of bench.py, to compare results with or as the timed CPU
+baseline.  The product path (hybridhsdla_amd) never imports it.
 """
 from __future__ import annotations
 
