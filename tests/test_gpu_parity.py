@@ -352,3 +352,25 @@ def test_autotune_picks_and_applies_a_tile(ctx):
     H, S, rep = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
     assert rep["tile_used"] == bi
     assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
+
+
+def test_tuning_knobs_keep_the_bits(ctx, monkeypatch):
+    """HSGEN_TAPP_TILE (another tile for the T application) and
+    HSGEN_RESIDUE_PASSES (batched split batches sized for one residue pass)
+    change speed only: every FP64 tile runs the same MFMA sequence per
+    element and the residues are exact."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
+    kps = _kset(s, nk=3, seed=12)
+    h = ctx.upload(s)
+    H0, S0 = ctx.create_HS2(h, kps[0])
+    for t in ("12", "8", "3"):
+        monkeypatch.setenv("HSGEN_TAPP_TILE", t)
+        H1, S1 = ctx.create_HS2(h, kps[0])
+        assert np.array_equal(H0, H1) and np.array_equal(S0, S1), t
+    monkeypatch.delenv("HSGEN_TAPP_TILE")
+    ref, _ = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    monkeypatch.setenv("HSGEN_RESIDUE_PASSES", "1")
+    res, _ = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    for (Ha, Sa), (Hb, Sb) in zip(ref, res):
+        assert np.array_equal(Ha, Hb) and np.array_equal(Sa, Sb)
