@@ -24,7 +24,8 @@ LIB = LIBDIR / "libhsgen.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liborc.so"
 
-HIP_SOURCES = ["abgen.hip", "zgemm.hip", "ozaki.hip", "api.hip"]
+HIP_SOURCES = ["abgen.hip", "zgemm.hip", "ozaki.hip", "context.hip", "fp64_path.hip", "split_path.hip",
+               "tune_eig.hip"]
 # host-only set-up code (radial solver, T assembly), linked into the same library
 HOST_SOURCES = ["radial.cpp", "tmat.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]
@@ -95,11 +96,18 @@ HDF5_ROOT = Path(os.environ.get("HSGEN_HDF5_ROOT", "/opt/conda"))
 IO_LIB = LIBDIR / "libhsgen_io.so"
 
 
+def hdf5_available() -> bool:
+    """True when the HDF5 headers and shared library the dump I/O library
+    builds against are installed (HSGEN_HDF5_ROOT, default /opt/conda)."""
+    inc, lib = HDF5_ROOT / "include", HDF5_ROOT / "lib"
+    return (inc / "hdf5.h").exists() and bool(list(lib.glob("libhdf5.so*")))
+
+
 def build_io() -> Path | None:
     """libhsgen_io.so: the Fleur dump reader / H-S writer (host C++, HDF5),
     linked against libhsgen.so.  Skipped when no HDF5 is installed."""
     inc, lib = HDF5_ROOT / "include", HDF5_ROOT / "lib"
-    if not (inc / "hdf5.h").exists() or not list(lib.glob("libhdf5.so*")):
+    if not hdf5_available():
         print(f"HDF5 not found under {HDF5_ROOT}: libhsgen_io.so not built", file=sys.stderr)
         return None
     tmp = IO_LIB.with_suffix(".so.tmp")

@@ -17,7 +17,7 @@ import pytest
 from hybridhsdla_amd import host, make_config
 from oracle import pyoracle as po
 from oracle import radial as orad
-from tests.helpers import rel_fro_lower
+from tests.helpers import open_dump_or_fail, rel_fro_lower, require_dump_io
 
 FIXTURE = "tests/golden/two_atoms/fleur_dump.h5"
 
@@ -26,10 +26,7 @@ dump_mod = pytest.importorskip("hybridhsdla_amd.dump")
 
 @pytest.fixture(scope="module")
 def dump():
-    try:
-        d = dump_mod.FleurDump(FIXTURE)
-    except RuntimeError as e:          # libhsgen_io.so not built (no HDF5)
-        pytest.skip(str(e))
+    d = open_dump_or_fail(FIXTURE)
     yield d
     d.close()
 
@@ -115,10 +112,8 @@ def test_unpack_lower_order():
 def test_write_read_round_trip(tmp_path):
     s, kps = make_config(0)
     path = tmp_path / "cfg0.h5"
-    try:
-        dump_mod.write_dump(path, s, kps)
-    except RuntimeError as e:
-        pytest.skip(str(e))
+    require_dump_io()
+    dump_mod.write_dump(path, s, kps)
     with dump_mod.FleurDump(path) as d:
         assert d.info["has_final"] == 1
         s2 = d.system()
@@ -139,10 +134,8 @@ def test_written_tmat_blocks_rebuild_the_final_t(tmp_path):
     rounding of the +-1/2 on the T_ab diagonal."""
     s, kps = make_config(0)
     path = tmp_path / "cfg0.h5"
-    try:
-        dump_mod.write_dump(path, s, kps)
-    except RuntimeError as e:
-        pytest.skip(str(e))
+    require_dump_io()
+    dump_mod.write_dump(path, s, kps)
     with dump_mod.FleurDump(path) as d:
         for t in range(2):
             td = d.type_data(t)
@@ -157,11 +150,9 @@ def test_hs_store(tmp_path):
     H = np.arange(9).reshape(3, 3, order="F") * (1 + 0.5j)
     S = np.eye(3) * (2 - 1j)
     path = tmp_path / "out.h5"
-    try:
-        with dump_mod.HSStore(path) as st:
-            st.add(0, H, S)
-    except RuntimeError as e:
-        pytest.skip(str(e))
+    require_dump_io()
+    with dump_mod.HSStore(path) as st:
+        st.add(0, H, S)
     exe = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
     out = subprocess.run([exe, "-y", "-w", "0", "-m", "%.17g", "-d", "/matrix_hamilton/1", str(path)], capture_output=True,
                          text=True, check=True).stdout

@@ -159,14 +159,17 @@ def test_repeated_g_vectors(ctx, big_list):
 
 
 def test_batched_ragged_sizes_equal_single_calls(ctx, big_list):
+    """Host outputs, so that the empty k-point also goes through the
+    device-to-host copy of the batched path."""
     s, kp = big_list
-    kps = [_subset(kp, [7]), _subset(kp, np.arange(129)), _subset(kp, np.arange(300) * 2),
+    kps = [_subset(kp, [7]), _subset(kp, []), _subset(kp, np.arange(129)), _subset(kp, np.arange(300) * 2),
            _subset(kp, np.arange(2) + 100)]
     h = ctx.upload(s)
     res, _ = ctx.create_HS2_batched(h, kps)
     for (Hb, Sb), k in zip(res, kps):
         H1, S1 = ctx.create_HS2(h, k)
         assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
+    assert res[1][0].shape == (0, 0)
 
 
 def test_batched_split_ragged_sizes_equal_single_calls(ctx, big_list):

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import scipy.linalg as sl
 
-from tests.helpers import rel_fro_lower
+from tests.helpers import open_dump_or_fail, rel_fro_lower
 
 pytestmark = pytest.mark.gpu
 
@@ -34,11 +34,7 @@ def test_well_conditioned_problem(ctx):
 def test_fleur_two_atoms_spectrum(ctx):
     """Eigenvalues of the generated H and S against scipy on Fleur's own H
     and S (S has condition ~1e8 here, which sets the tolerance)."""
-    from hybridhsdla_amd import dump as dump_mod
-    try:
-        d = dump_mod.FleurDump("tests/golden/two_atoms/fleur_dump.h5")
-    except (RuntimeError, OSError) as e:
-        pytest.skip(f"dump reader unavailable: {e}")
+    d = open_dump_or_fail("tests/golden/two_atoms/fleur_dump.h5")
     s, kp = d.system(), d.kpoint(0)
     Hf, Sf = d.fleur_hs(0)
     d.close()
