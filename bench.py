@@ -11,6 +11,10 @@ the system description already resident in HBM and H, S written to HBM.
 Prints ONE JSON line.  The default workload is the 256-atom, lmax 12,
 N_G ~ 25000 system (FP64).  Effective TFLOP/s uses the algorithmic work
 F = 16 G^2 K + 32 G sum_a L_a^2 (complex MAC = 8 flops, lower triangle).
+After the FP64 run the same k-point runs in split precision at the targets
+1e-10, 1e-12 (the headline, timed like the FP64 run) and 1e-13; the CPU leg
+times the reference CPU path and checks every result against the oracle on a
+stratified sample of 196 G-vectors.
 """
 from __future__ import annotations
 
@@ -41,23 +45,43 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_threads() -> int:
+def cpu_threads():
+    """(threads used, CPUs in the affinity mask): OMP_NUM_THREADS where it is
+    set (the host's CPU share for this GPU), else every CPU of the mask."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    try:
+        t = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        t = 0
+    return (min(t, n) if t > 0 else n), n
 
 
-def cpu_baseline(cfg: int, budget_s: float):
-    """Time the reference CPU path (oracle: serial construct_AB + the v2 BLAS
-    sequence on the host OpenBLAS) on the same system at a reduced plane-wave
-    count sized to ~budget_s of CPU work."""
+def stratified_sample(ng, block=128, seed=1):
+    """One index drawn from every block of `block` consecutive G-vectors (the
+    same sample the full-size parity tests use)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.array([b0 + int(rng.integers(0, min(block, ng - b0))) for b0 in range(0, ng, block)], dtype=np.int64)
+
+
+def cpu_baseline(cfg: int, budget_s: float, samples=None):
+    """The reference CPU path (oracle: serial construct_AB, tensor_AB_match.cpp:63-129,
+    then the v2 BLAS sequence, h_and_s_stripped_v2.cpp:100-382, on the host
+    OpenBLAS), timed stage by stage on the same system at a reduced plane-wave
+    count sized to ~budget_s, each stage extrapolated to the full size by its
+    own law (construct_AB ~ G, the BLAS sequence ~ 20 G^2 K + 24 G sum L^2);
+    for cfg >= 2 also the medium workload (32 atoms, N_G 6002) timed whole at
+    full size.  `samples` maps labels to (idx, H_sub, S_sub), principal
+    submatrices of the GPU results on a stratified G sample; each is compared
+    with the oracle evaluated on those G-vectors."""
     import numpy as np
     from hybridhsdla_amd.system import gvectors, kmax_for_ng, make_config, KPoint
     from oracle import pyoracle as po
 
-    threads = cpu_threads()
+    threads, ncpu = cpu_threads()
     if not po.blas_ready(threads):
         return None
     system, kps = make_config(cfg)
@@ -65,38 +89,86 @@ def cpu_baseline(cfg: int, budget_s: float):
     full_ng = kp0.ng
     k = kp0.k
     b33 = system.bmat33()
+    s = system
 
-    def run(ng):
-        km = kmax_for_ng(k, b33, ng)
-        kp = KPoint(k=k, g=gvectors(k, b33, km), kmax=km)
+    def timed(sysm, kp):
         t0 = time.perf_counter()
-        po.reference_hs(system, kp, threads=threads)
-        return time.perf_counter() - t0, kp.ng
+        A, B = po.construct_ab(sysm, kp)
+        t1 = time.perf_counter()
+        po.create_hs2_v2(sysm.L, sysm.natoms, kp.ng, A, B, sysm.taa, sysm.tab, sysm.tbb, sysm.lmax, sysm.max_lmax,
+                         sysm.sqrt_u_norms(), threads=threads)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    def sub_kp(ng):
+        km = kmax_for_ng(k, b33, ng)
+        return KPoint(k=k, g=gvectors(k, b33, km), kmax=km)
 
     # calibrate on a small sample, then size the real sample to the budget
-    t_small, ng_small = run(min(400, full_ng))
-    f_small = system.flops_alg(ng_small)
-    rate = f_small / max(t_small, 1e-6)
-    ng_s = ng_small
+    kp_small = sub_kp(min(400, full_ng))
+    ta, tc = timed(s, kp_small)
+    rate = s.flops_ref(kp_small.ng) / max(tc, 1e-6)
+    ab_per_g = ta / kp_small.ng
+    ng_s = kp_small.ng
     for cand in (800, 1200, 1600, 2000, 2500, 3000, 4000, 5000, 6000, 8000):
         if cand > full_ng:
             break
-        if system.flops_alg(cand) / rate <= budget_s:
+        if s.flops_ref(cand) / rate + ab_per_g * cand <= budget_s:
             ng_s = cand
-    t_s, ng_real = run(ng_s) if ng_s != ng_small else (t_small, ng_small)
-    f_s = system.flops_alg(ng_real)
-    tflops = f_s / t_s / 1e12
-    return {
-        "value": round(tflops, 5),
+    if ng_s != kp_small.ng:
+        kp_s = sub_kp(ng_s)
+        ta, tc = timed(s, kp_s)
+    else:
+        kp_s = kp_small
+    g_s = kp_s.ng
+    ext_ab = ta * full_ng / g_s
+    ext_c = tc * s.flops_ref(full_ng) / s.flops_ref(g_s)
+    sec_full = ext_ab + ext_c
+    out = {
+        "value": round(s.flops_alg(full_ng) / sec_full / 1e12, 5),
         "unit": "TFLOP/s",
         "cores": threads,
+        "host_cpus_in_affinity_mask": ncpu,
         "kind": "port",
-        "sample": (f"{WORKLOAD.get(cfg, cfg)} system at N_G={ng_real} of {full_ng} "
-                   f"(smaller |k+G| cutoff); serial construct_AB + v2 BLAS sequence "
-                   f"(OpenBLAS, {threads} threads); {t_s:.2f} s"),
-        "sec_sample": round(t_s, 4),
-        "sec_per_hs_extrapolated": round(system.flops_alg(full_ng) / (tflops * 1e12), 2),
+        "sample": (f"{WORKLOAD.get(cfg, cfg)} system at N_G={g_s} of {full_ng} (smaller |k+G| cutoff): "
+                   f"serial construct_AB {ta:.2f} s + v2 BLAS sequence {tc:.2f} s (OpenBLAS, {threads} threads); "
+                   f"value = F_alg at N_G={full_ng} over the extrapolated time"),
+        "ng_sample": g_s,
+        "sec_construct_ab": round(ta, 4),
+        "sec_contract": round(tc, 4),
+        "sec_construct_ab_extrapolated": round(ext_ab, 2),
+        "sec_contract_extrapolated": round(ext_c, 2),
+        "sec_per_hs_extrapolated": round(sec_full, 2),
+        "extrapolation": "construct_AB x G/G_s; BLAS sequence x F_ref(G)/F_ref(G_s), F_ref = 20 G^2 K + 24 G sum L_a^2",
     }
+    if cfg >= 2:
+        s1, k1 = make_config(1)
+        ta1, tc1 = timed(s1, k1[0])
+        out["measured_medium_full_size"] = {
+            "workload": WORKLOAD[1] + f", N_G={k1[0].ng}, K={s1.kdim}",
+            "sec_per_hs": round(ta1 + tc1, 3), "sec_construct_ab": round(ta1, 3), "sec_contract": round(tc1, 3),
+            "tflops": round(s1.flops_alg(k1[0].ng) / (ta1 + tc1) / 1e12, 5), "threads": threads,
+        }
+    if samples:
+        checks = {}
+        for label, (idx, Hs, Ss) in samples.items():
+            key = tuple(idx.tolist())
+            if key not in checks:
+                sub = KPoint(k=kp0.k, g=kp0.g[idx], kmax=kp0.kmax)
+                checks[key] = po.reference_hs(s, sub, threads=threads)
+            Hr, Sr = checks[key]
+            rel = lambda x, r: float(np.linalg.norm(np.tril(x - r)) / np.linalg.norm(np.tril(r)))
+            out.setdefault("oracle_sample_check", {})[label] = {
+                "n_sample": int(len(idx)), "rel_err_H": rel(Hs, Hr), "rel_err_S": rel(Ss, Sr)}
+    return out
+
+
+def gather_sample(H, S, idx):
+    """Principal submatrices of device H, S (column-major in torch tensors) on
+    the indices idx, as column-major numpy arrays."""
+    import torch
+    ti = torch.as_tensor(idx, device=H.device)
+    return H[ti][:, ti].cpu().numpy().T.copy(), S[ti][:, ti].cpu().numpy().T.copy()
 
 
 def self_check(ctx, sysh, kp, H_dev, S_dev, nsample=48, seed=1):
@@ -144,39 +216,40 @@ def lower_rel_err(X, ref, rows=2048):
     return (num / den) ** 0.5 if den > 0 else 0.0
 
 
-def split_run(args, system, G, H, S, step, fp64_ms, F):
-    """Split precision (INT8 Ozaki) on the same k-point at the stated error
-    target, timed like the FP64 run, with its error against the FP64 result."""
+def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, warmup, idx):
+    """Split precision (INT8 Ozaki) on the same k-point at the error target,
+    timed like the FP64 run, with its error against the FP64 result on the
+    whole lower triangle; returns the entry and the result's principal
+    submatrices on the stratified sample idx."""
     import numpy as np
     import torch
     from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
 
-    H64 = H.clone()
-    S64 = S.clone()
-    for _ in range(args.warmup):
-        step(HSGEN_PREC_OZAKI_INT8, args.split_target)
+    for _ in range(warmup):
+        step(HSGEN_PREC_OZAKI_INT8, target)
     torch.cuda.synchronize()
     reps = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        reps.append(step(HSGEN_PREC_OZAKI_INT8, args.split_target))
+    for _ in range(steps):
+        reps.append(step(HSGEN_PREC_OZAKI_INT8, target))
     torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
     eH = lower_rel_err(H, H64)
     eS = lower_rel_err(S, S64)
-    del H64, S64
+    smp = gather_sample(H, S, idx)
     r = reps[-1]
     nmod = int(r["slices_used"])
     ms_ig = float(np.mean([x["ms_igemm"] for x in reps]))
     ops = nmod * 16.0 * G * G * system.kdim      # INT8 ops of the nmod modulus GEMMs
     ach = ops / (ms_ig * 1e-3) / 1e12 if ms_ig > 0 else None
-    return {
+    entry = {
+        "target_rel_err": target,
+        "moduli": nmod,
+        "steps": steps, "warmup": warmup,
         "sec_per_hs": round(ms * 1e-3, 4),
         "value": round(F / (ms * 1e-3) / 1e12, 3),
         "unit": "TFLOP/s (FP64-equivalent)",
         "speedup_vs_fp64": round(fp64_ms / ms, 3),
-        "target_rel_err": args.split_target,
-        "moduli": nmod,
         "rel_err_H_vs_fp64": eH,
         "rel_err_S_vs_fp64": eS,
         "err_model_H": r["err_est_h"], "err_model_S": r["err_est_s"],
@@ -192,6 +265,7 @@ def split_run(args, system, G, H, S, step, fp64_ms, F):
             "avg_launch_ms": round(ms_ig / max(nmod, 1), 3),
         },
     }
+    return entry, smp
 
 
 def bench_batched(args) -> int:
@@ -330,7 +404,10 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split-precision comparison run after the FP64 run")
-    ap.add_argument("--split-target", type=float, default=1e-12, help="split precision: relative error target")
+    ap.add_argument("--split-target", type=float, default=1e-12,
+                    help="split precision: relative error target of the headline run")
+    ap.add_argument("--split-sweep", default="1e-10,1e-12,1e-13",
+                    help="split precision: further targets, timed with at most 3 steps each")
     ap.add_argument("--eig", action="store_true",
                     help="also time the generalized eigensolve H x = lambda S x of the result")
     ap.add_argument("--host-out", action="store_true",
@@ -469,21 +546,45 @@ def main(argv=None) -> int:
         del Hh, Sh
         log(f"[bench] host outputs: {host_out}")
 
+    # principal submatrices on a stratified G sample, for the oracle check of
+    # the CPU leg
+    idx = stratified_sample(G)
+    samples = {"fp64": (idx, *gather_sample(H, S, idx))} if prec == HSGEN_PREC_FP64 else {}
     split = None
+    sweep = None
     if prec == HSGEN_PREC_FP64 and not args.no_split:
         try:
-            split = split_run(args, system, G, H, S, step, ms_step, F)
-            log(f"[bench] split precision: {split}")
+            H64 = H.clone()
+            S64 = S.clone()
+            sweep = []
+            targets = sorted({float(t) for t in args.split_sweep.split(",")} | {args.split_target}, reverse=True)
+            for target in targets:
+                main_t = target == args.split_target
+                # the headline target is timed like the FP64 run, the others
+                # with fewer steps
+                st, wu = (args.steps, args.warmup) if main_t else (min(args.steps, 3), min(args.warmup, 1))
+                entry, smp = split_run(args, system, G, H, S, H64, S64, step, ms_step, F, target, st, wu, idx)
+                samples[f"split_{target:.0e}"] = (idx, *smp)
+                sweep.append(entry)
+                if main_t:
+                    split = entry
+                log(f"[bench] split precision {target:.0e}: {entry['moduli']} moduli, {entry['sec_per_hs']} s, "
+                    f"{entry['speedup_vs_fp64']}x, err H {entry['rel_err_H_vs_fp64']:.2e} S "
+                    f"{entry['rel_err_S_vs_fp64']:.2e}")
+            del H64, S64
         except Exception as e:
             split = {"error": str(e)}
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args.config, args.cpu_budget)
+            cpu = cpu_baseline(args.config, args.cpu_budget, samples)
             log(f"[bench] cpu baseline: {cpu}")
         except Exception as e:
             cpu = {"error": str(e)}
+    if cpu and sweep:
+        for e in sweep:
+            e["rel_err_vs_oracle_sample"] = cpu.get("oracle_sample_check", {}).get(f"split_{e['target_rel_err']:.0e}")
 
     out = {
         "metric": METRIC,
@@ -511,6 +612,7 @@ def main(argv=None) -> int:
         "abgen_roofline": abgen,
         "self_check": check,
         "split_precision": split,
+        "split_sweep": sweep,
         "host_outputs": host_out,
         "eigensolve": eig,
         "cpu_baseline": cpu,
