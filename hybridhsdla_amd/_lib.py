@@ -89,6 +89,9 @@ SIGNATURES = [
     ("hsgen_create_hs", C.c_int,
      [_p, _p, C.POINTER(HsgenKpoint), C.POINTER(HsgenOpts), _p, _p, C.c_int64,
       C.POINTER(HsgenReport)]),
+    ("hsgen_create_hs_noalloc", C.c_int,
+     [_p, _p, C.POINTER(HsgenKpoint), C.POINTER(HsgenOpts), _p, _p, C.c_int64,
+      C.POINTER(HsgenReport)]),
     ("hsgen_create_hs_batched", C.c_int,
      [_p, _p, C.c_int32, C.POINTER(HsgenKpoint), C.POINTER(HsgenOpts), C.POINTER(_p),
       C.POINTER(_p), C.POINTER(C.c_int64), C.POINTER(HsgenReport)]),

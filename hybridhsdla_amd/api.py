@@ -179,6 +179,17 @@ class Context:
                                             s_ptr, int(ld), C.byref(rep)), self.handle)
         return rep.as_dict()
 
+    def create_hs_noalloc(self, sysh: SystemHandle, kp: KPoint, h_ptr: int, s_ptr: int, ld: int, flags=0,
+                          tile=0):
+        """hsgen_create_hs_noalloc: FP64 H and S into device pointers with no
+        device allocation (the library's fixed per-GPU scratch array)."""
+        ks, g = _kpoint(kp)
+        o = _opts(HSGEN_PREC_FP64, 0, 0.0, flags, tile)
+        rep = HsgenReport()
+        _lib.check(self.lib.hsgen_create_hs_noalloc(self.handle, sysh.handle, C.byref(ks), C.byref(o), h_ptr,
+                                                    s_ptr, int(ld), C.byref(rep)), self.handle)
+        return rep.as_dict()
+
     def autotune(self, sysh: SystemHandle, kp: KPoint, precision=HSGEN_PREC_FP64):
         """Time the tile configurations on kp, remember the fastest for this
         problem size; returns (config id, contraction ms)."""

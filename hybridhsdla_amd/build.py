@@ -25,7 +25,7 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liborc.so"
 
 HIP_SOURCES = ["abgen.hip", "zgemm.hip", "ozaki.hip", "context.hip", "fp64_path.hip", "split_path.hip",
-               "tune_eig.hip"]
+               "tune_eig.hip", "noalloc.hip"]
 # host-only set-up code (radial solver, T assembly), linked into the same library
 HOST_SOURCES = ["radial.cpp", "tmat.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]

@@ -107,6 +107,7 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
     if (s->d_M) {
         (void)hipFree(s->d_M);
         s->d_M = nullptr;
+        s->d_mloc = nullptr;
     }
     const int na = s->natoms;
     s->m_off.assign(na, 0);
@@ -152,8 +153,16 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
                 M[(n + r) + ld * c] = std::conj(Tab[c + (int64_t)n * r]);   // T_ab^H block
             }
     }
-    HIPCHK(hipMalloc(&s->d_M, sizeof(double2) * hM.size()));
+    // the per-atom layout table (offset, leading dimension) follows the
+    // blocks in the same allocation, for kernels that build their own work
+    // lists
+    const size_t mbytes = (size_t)roundup((int64_t)(sizeof(double2) * hM.size()), 256);
+    std::vector<MLoc> loc(na);
+    for (int a = 0; a < na; ++a) loc[a] = MLoc{s->m_off[a], s->m_ld[a], s->m_cols[a]};
+    HIPCHK(hipMalloc(&s->d_M, mbytes + sizeof(MLoc) * na));
     HIPCHK(hipMemcpy(s->d_M, hM.data(), sizeof(double2) * hM.size(), hipMemcpyHostToDevice));
+    s->d_mloc = (MLoc*)((char*)s->d_M + mbytes);
+    HIPCHK(hipMemcpy(s->d_mloc, loc.data(), sizeof(MLoc) * na, hipMemcpyHostToDevice));
     s->m_bk = bk;
     s->m_bm = bm;
     return HSGEN_OK;
@@ -313,6 +322,11 @@ hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
     for (auto& ev : ctx->oz_ev) HIPCHK(hipEventCreate(&ev));
+    const hsgen_status st = noalloc_setup(ctx);
+    if (st != HSGEN_OK) {
+        hsgen_finalize(ctx);
+        return st;
+    }
     *out = ctx;
     return HSGEN_OK;
 }

@@ -755,7 +755,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     if (st) return st;
     struct Guard {
         hsgen_sys* s;
-        ~Guard() { if (s->d_M) (void)hipFree(s->d_M); s->d_M = nullptr; }
+        ~Guard() { if (s->d_M) (void)hipFree(s->d_M); s->d_M = nullptr; s->d_mloc = nullptr; }
     } guard{&s};
     // device copies of A, B if they are host memory
     const size_t nab = (size_t)L * a * G;

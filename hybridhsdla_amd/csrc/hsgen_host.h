@@ -5,6 +5,7 @@
 //   split_path.hip  split precision (Ozaki scheme II on INT8 MFMA), single-k
 //                   and batched
 //   tune_eig.hip    the tile autotuner and the generalized eigensolve
+//   noalloc.hip     the allocation-free FP64 entry point
 //
 // Operand layout in HBM, per k-point (complex128, column-major, ld = ldz):
 //   Z  = [A_1; B_1; A_2; B_2; ...]   rows 2*off_a .. 2*off_a + 2*nlm_a - 1
@@ -36,6 +37,13 @@ constexpr int kSampleTile = 12;    // FP64 tile of the split-precision sample: 6
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+};
+
+// Where atom a's M_a block lives in the system's device allocation.
+struct MLoc {
+    int64_t off;       // offset of M_a (complex units)
+    int32_t ld;        // leading dimension (2 nlm_a rounded up to the tile's BK)
+    int32_t cols;      // padded column count (2 nlm_a rounded up to the tile's BM)
 };
 
 }  // namespace hsgen
@@ -85,6 +93,7 @@ struct hsgen_sys {
     hsgen::RadDev* d_rad = nullptr;
     double* d_ynorm = nullptr;
     double2* d_M = nullptr;
+    hsgen::MLoc* d_mloc = nullptr;     // [natoms] M_a layout, in the same allocation as d_M
     // host copies of the T blocks, so that M can be rebuilt for another tile
     // geometry
     std::vector<std::vector<std::complex<double>>> h_taa, h_tab, h_tbb;
@@ -170,6 +179,10 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
                        int nmod_req, double target, std::vector<OzResult>& res);
 hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoint* kps, const hsgen_opts& o,
                            double* const* H, double* const* S, const int64_t* ld, hsgen_report* rep);
+
+// ---- noalloc.hip ----
+// Look up this context's device's scratch array once (the device current).
+hsgen_status noalloc_setup(hsgen_ctx* ctx);
 
 // ---- tune_eig.hip ----
 // Size buckets of the autotuning table: half-octaves of G and K.

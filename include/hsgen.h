@@ -203,6 +203,28 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G,
                                const hsgen_opts* opts, double* H, double* S,
                                int64_t ld, hsgen_report* report);
 
+/* Bytes of device scratch the allocation-free entry point below has per GPU. */
+#define HSGEN_NOALLOC_SCRATCH_BYTES ((uint64_t)256 << 20)
+
+/* hsgen_create_hs in FP64 with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * call's workspaces (a host G list's device copy, the grouped-GEMM work
+ * lists, the operands Z, Zs, W: 3 x 16 B x (2K + 32) x roundup(ng, 128) for
+ * the default tile) live in one device array of HSGEN_NOALLOC_SCRATCH_BYTES
+ * per GPU, held by the library until the process ends; a problem that needs
+ * more is HSGEN_ERR_ARG.  H and S must be device memory (HSGEN_ERR_ARG
+ * otherwise); kp->g may be host or device memory.  opts->precision must be
+ * HSGEN_PREC_FP64 (HSGEN_ERR_UNSUPPORTED otherwise); flags and tile as for
+ * hsgen_create_hs, with the tile's T-block padding already in place (the
+ * default tile's is made by hsgen_system_create).  Results equal
+ * hsgen_create_hs's bit for bit.  Calls on one GPU share its array: calls
+ * that overlap on one GPU, from any contexts, race. */
+hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys* sys,
+                                     const hsgen_kpoint* kp, const hsgen_opts* opts,
+                                     double* H, double* S, int64_t ld,
+                                     hsgen_report* report);
+
 /* The step after H/S: the generalized Hermitian eigenproblem H x = lambda S x
  * of one k-point.  No counterpart in the reference, which stops at H and S;
  * Fleur hands them to its own solvers (eigen.F:750,781).  H and S: n x n
