@@ -19,8 +19,9 @@
 // Output layouts:
 //   compact (create_hs): Z[col*ldz + 2*off_a + r], r < nlm_a -> A_a(lm=r),
 //             r >= nlm_a -> B_a(lm=r-nlm_a); Zs the same with B rows scaled
-//             by ||u_dot_l||^2 (S = Z^H Zs).  Rows >= 2K and columns >= ng are
-//             written as zeros (tile padding).
+//             by N_l = ||u_dot_l||^2 (S = Z^H Zs).  Folded (fold != 0): Z alone
+//             with B rows scaled by sqrt(N_l) (S = Z^H Z).  Rows >= 2K and
+//             columns >= ng are written as zeros (tile padding).
 //   ref:      A[col*L*natoms + a*L + lm], B likewise, rows lm >= nlm_a zero
 //             (the reference's arma::cx_cube(L, natoms, G) layout).
 #include "hsgen_internal.h"
@@ -114,8 +115,11 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
         // padding column: zeros
         double2 z = make_double2(0.0, 0.0);
         double2* zc = p.Z + (int64_t)col * p.ldz;
-        double2* zsc = p.Zs + (int64_t)col * p.ldz;
-        for (int64_t r = tid; r < p.ldz; r += blockDim.x) { zc[r] = z; zsc[r] = z; }
+        for (int64_t r = tid; r < p.ldz; r += blockDim.x) zc[r] = z;
+        if (!p.fold) {
+            double2* zsc = p.Zs + (int64_t)col * p.ldz;
+            for (int64_t r = tid; r < p.ldz; r += blockDim.x) zsc[r] = z;
+        }
         return;
     }
 
@@ -240,9 +244,14 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
                 const double2 y = s_yl[t * L + lm];
                 const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
                 double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
-                p.Z[base + q] = v;
-                double sc = isB ? p.rad[(int64_t)(a0 + t) * nl + l].dE_u_norm : 1.0;
-                p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
+                if (p.fold) {
+                    const double sc = isB ? p.rad[(int64_t)(a0 + t) * nl + l].sqrt_un : 1.0;
+                    p.Z[base + q] = make_double2(v.x * sc, v.y * sc);
+                } else {
+                    p.Z[base + q] = v;
+                    double sc = isB ? p.rad[(int64_t)(a0 + t) * nl + l].dE_u_norm : 1.0;
+                    p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
+                }
             }
         } else {
             const int64_t ld = (int64_t)L * p.natoms;
@@ -270,7 +279,7 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
         double2 z = make_double2(0.0, 0.0);
         for (int64_t r = r0 + tid; r < p.ldz; r += blockDim.x) {
             p.Z[(int64_t)col * p.ldz + r] = z;
-            p.Zs[(int64_t)col * p.ldz + r] = z;
+            if (!p.fold) p.Zs[(int64_t)col * p.ldz + r] = z;
         }
     }
 }
@@ -353,17 +362,17 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
     double* s_bes = (double*)(s_y + p.nrot * L);         // [nrmt][2 nl] fj, dfj
     double* s_dir = s_bes + p.nrmt * 2 * nl;             // [nrot][4]    cth sth cph sph
     double* s_rf = s_dir + p.nrot * 4;                   // [chunk][nl][4]
-    double* s_un = s_rf + chunk * nl * 4;                // [chunk][nl]  Zs scale of B rows
+    double* s_un = s_rf + chunk * nl * 4;                // [chunk][nl]  B-row scale (Zs, or Z folded)
     int* s_off = (int*)(s_un + chunk * nl);              // [chunk+1]    row prefix (compact)
     int* s_rc = s_off + chunk + 1;                       // [chunk]      rotation class
 
     if (kCompact && col >= p.ng) {
         const double2 z = make_double2(0.0, 0.0);
         double2* zc = p.Z + (int64_t)col * p.ldz;
-        double2* zsc = p.Zs + (int64_t)col * p.ldz;
-        for (int64_t r = tid; r < p.ldz; r += blockDim.x) {
-            zc[r] = z;
-            zsc[r] = z;
+        for (int64_t r = tid; r < p.ldz; r += blockDim.x) zc[r] = z;
+        if (!p.fold) {
+            double2* zsc = p.Zs + (int64_t)col * p.ldz;
+            for (int64_t r = tid; r < p.ldz; r += blockDim.x) zsc[r] = z;
         }
         return;
     }
@@ -450,7 +459,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
             rf[1] = ci * Au;
             rf[2] = cr * Bu;
             rf[3] = ci * Bu;
-            s_un[t * nl + l] = m.dE_u_norm;
+            s_un[t * nl + l] = p.fold ? m.sqrt_un : m.dE_u_norm;
         }
         if (kCompact && tid == 0) {
             int acc = 0;
@@ -475,9 +484,13 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
                 const double2 y = s_y[s_rc[t] * L + lm];
                 const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
                 const double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
-                p.Z[base + q] = v;
                 const double sc = isB ? s_un[t * nl + l] : 1.0;
-                p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
+                if (p.fold) {
+                    p.Z[base + q] = make_double2(v.x * sc, v.y * sc);
+                } else {
+                    p.Z[base + q] = v;
+                    p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
+                }
             }
         } else {
             const int64_t ld = (int64_t)L * p.natoms;
@@ -504,7 +517,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
         const double2 z = make_double2(0.0, 0.0);
         for (int64_t r = r0 + tid; r < p.ldz; r += blockDim.x) {
             p.Z[(int64_t)col * p.ldz + r] = z;
-            p.Zs[(int64_t)col * p.ldz + r] = z;
+            if (!p.fold) p.Zs[(int64_t)col * p.ldz + r] = z;
         }
     }
 }
@@ -548,9 +561,12 @@ __global__ __launch_bounds__(256) void k_pack_ref(PackParams p)
     const int col = blockIdx.x;
     const double2 zero = make_double2(0.0, 0.0);
     double2* zc = p.Z + (int64_t)col * p.ldz;
-    double2* zsc = p.Zs + (int64_t)col * p.ldz;
+    double2* zsc = p.fold ? nullptr : p.Zs + (int64_t)col * p.ldz;
     if (col >= p.ng) {
-        for (int64_t r = threadIdx.x; r < p.ldz; r += blockDim.x) { zc[r] = zero; zsc[r] = zero; }
+        for (int64_t r = threadIdx.x; r < p.ldz; r += blockDim.x) {
+            zc[r] = zero;
+            if (zsc) zsc[r] = zero;
+        }
         return;
     }
     // walk atoms: each atom contributes 2*nlm rows
@@ -562,18 +578,22 @@ __global__ __launch_bounds__(256) void k_pack_ref(PackParams p)
             const int lm = isB ? r - nlm : r;
             const int64_t src = (int64_t)col * p.ldab + (int64_t)a * p.L + lm;
             double2 v = isB ? p.B[src] : p.A[src];
-            zc[2 * (int64_t)at.off + r] = v;
             double sc = 1.0;
             if (isB) {
                 const int l = (int)sqrt((double)lm + 0.5);
-                sc = p.unorm2[a * p.nl + l];
+                sc = p.scale[a * p.nl + l];
             }
-            zsc[2 * (int64_t)at.off + r] = make_double2(v.x * sc, v.y * sc);
+            if (zsc) {
+                zc[2 * (int64_t)at.off + r] = v;
+                zsc[2 * (int64_t)at.off + r] = make_double2(v.x * sc, v.y * sc);
+            } else {
+                zc[2 * (int64_t)at.off + r] = make_double2(v.x * sc, v.y * sc);
+            }
         }
     }
     for (int64_t r = 2 * (int64_t)p.kdim + threadIdx.x; r < p.ldz; r += blockDim.x) {
         zc[r] = zero;
-        zsc[r] = zero;
+        if (zsc) zsc[r] = zero;
     }
 }
 

@@ -98,11 +98,26 @@ std::vector<double> ynorm_table(int lmax)
 
 }  // namespace
 
+bool fold_enabled()
+{
+    const char* e = std::getenv("HSGEN_FOLD");
+    return !(e && e[0] == '0');
+}
+
+bool fold_ok(double n)
+{
+    // 1 / sqrt(N) and products of two of them stay far from overflow
+    return n > 1e-150 && n < 1e150;
+}
+
 // Build the per-atom M_a = [[T_aa, T_ab], [T_ab^H, T_bb]] (T_aa, T_bb taken
 // from their lower triangles) padded for tile geometry (bk, bm), deduplicated
-// by T pointer triple, and upload.
+// by T pointer triple (and B-row scales), and upload.  With rsc (folded
+// operands, rsc[a * nl + l] = sqrt(N_l) of atom a) the block is
+// M'_a = D^-1/2 M_a D^-1/2, D = diag(1 on A rows, N_l on B rows), so that
+// Z^H M Z = Y^H M' Y with Y = D^1/2 Z.
 hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const double* const* taa,
-                      const double* const* tab, const double* const* tbb, int bk, int bm)
+                      const double* const* tab, const double* const* tbb, const double* rsc, int bk, int bm)
 {
     if (s->d_M) {
         (void)hipFree(s->d_M);
@@ -110,14 +125,17 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
         s->d_mloc = nullptr;
     }
     const int na = s->natoms;
+    const int nl = s->nl;
     s->m_off.assign(na, 0);
     s->m_ld.assign(na, 0);
     s->m_cols.assign(na, 0);
-    std::map<std::vector<const void*>, int> seen;
+    std::map<std::pair<std::vector<const void*>, std::vector<double>>, int> seen;
     std::vector<int> owner(na, -1);
     int64_t total = 0;
     for (int a = 0; a < na; ++a) {
-        std::vector<const void*> key{taa[a], tab[a], tbb[a], (const void*)(intptr_t)nlm[a]};
+        std::pair<std::vector<const void*>, std::vector<double>> key{
+            {taa[a], tab[a], tbb[a], (const void*)(intptr_t)nlm[a]},
+            rsc ? std::vector<double>(rsc + (size_t)a * nl, rsc + (size_t)(a + 1) * nl) : std::vector<double>()};
         auto it = seen.find(key);
         const int n2 = 2 * nlm[a];
         s->m_ld[a] = (int32_t)roundup(n2, bk);
@@ -133,6 +151,7 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
         total += (int64_t)s->m_ld[a] * s->m_cols[a];
     }
     std::vector<std::complex<double>> hM((size_t)std::max<int64_t>(total, 1), 0.0);
+    std::vector<double> sq;
     for (int a = 0; a < na; ++a) {
         if (owner[a] != a) continue;
         const int n = nlm[a];
@@ -141,16 +160,32 @@ hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const do
         const std::complex<double>* Taa = (const std::complex<double>*)taa[a];
         const std::complex<double>* Tab = (const std::complex<double>*)tab[a];
         const std::complex<double>* Tbb = (const std::complex<double>*)tbb[a];
+        // B-row scale per lm (1 without folding)
+        sq.assign(n, 1.0);
+        if (rsc)
+            for (int lm = 0; lm < n; ++lm) {
+                int l = (int)std::sqrt((double)lm + 0.5);
+                while ((l + 1) * (l + 1) <= lm) ++l;
+                while (l * l > lm) --l;
+                sq[lm] = rsc[(size_t)a * nl + l];
+            }
         for (int c = 0; c < n; ++c)
             for (int r = 0; r < n; ++r) {
                 // Hermitian expansion from the lower triangle
                 std::complex<double> aa = r >= c ? Taa[r + (int64_t)n * c] : std::conj(Taa[c + (int64_t)n * r]);
                 std::complex<double> bb = r >= c ? Tbb[r + (int64_t)n * c] : std::conj(Tbb[c + (int64_t)n * r]);
                 if (r == c) { aa = aa.real(); bb = bb.real(); }
+                std::complex<double> ab = Tab[r + (int64_t)n * c];              // T_ab block
+                std::complex<double> ba = std::conj(Tab[c + (int64_t)n * r]);   // T_ab^H block
+                if (rsc) {
+                    ab /= sq[c];
+                    ba /= sq[r];
+                    bb /= sq[r] * sq[c];
+                }
                 M[r + ld * c] = aa;
                 M[(n + r) + ld * (n + c)] = bb;
-                M[r + ld * (n + c)] = Tab[r + (int64_t)n * c];              // T_ab block
-                M[(n + r) + ld * c] = std::conj(Tab[c + (int64_t)n * r]);   // T_ab^H block
+                M[r + ld * (n + c)] = ab;
+                M[(n + r) + ld * c] = ba;
             }
     }
     // the per-atom layout table (offset, leading dimension) follows the
@@ -181,7 +216,8 @@ hsgen_status ensure_M(hsgen_ctx* ctx, hsgen_sys* s, int bk, int bm)
         pc[a] = (const double*)s->h_tbb[o].data();
         nlm[a] = s->atoms[a].nlm;
     }
-    return upload_M(ctx, s, nlm.data(), pa.data(), pb.data(), pc.data(), bk, bm);
+    return upload_M(ctx, s, nlm.data(), pa.data(), pb.data(), pc.data(), s->fold ? s->h_sqrtn.data() : nullptr,
+                    bk, bm);
 }
 
 hsgen_status check_kpoint(hsgen_ctx* ctx, const hsgen_kpoint* kp)
@@ -396,6 +432,8 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
     int off = 0;
     s->atoms.resize(d->natoms);
     std::vector<RadDev> rad((size_t)d->natoms * s->nl);
+    s->h_sqrtn.assign((size_t)d->natoms * s->nl, 0.0);
+    bool fold = fold_enabled();
     for (int a = 0; a < d->natoms; ++a) {
         const int lm = d->lmax[a];
         if (lm < 0 || lm > d->max_lmax) {
@@ -438,12 +476,16 @@ hsgen_status hsgen_system_create(hsgen_ctx* ctx, const hsgen_system* d, hsgen_sy
                 r.dE_u_dr = m[3];
                 r.dE_u_norm = m[4];
                 r.inv_wronk = 1.0 / (r.dE_u * r.u_dr - r.u * r.dE_u_dr);   // AtomData.cpp:63-66
+                r.sqrt_un = std::sqrt(r.dE_u_norm);                        // h_and_s.cpp:710
+                if (!fold_ok(r.dE_u_norm)) fold = false;
             } else {
-                r = RadDev{0, 0, 0, 0, 0, 0};
+                r = RadDev{0, 0, 0, 0, 0, 0, 0};
             }
+            s->h_sqrtn[(size_t)a * s->nl + l] = r.sqrt_un;
         }
     }
     s->kdim = off;
+    s->fold = fold;
     auto yn = ynorm_table(s->max_lmax);
     hsgen_status st = HSGEN_OK;
     hipError_t e = hipSuccess;

@@ -32,6 +32,7 @@ int64_t int8_col_align(int tile)
 Geometry geometry(int tile, int kdim, int ng, int64_t col_align)
 {
     Geometry g;
+    g.fold = false;
     g.cfg = tile > 0 ? tile : kDefaultTile;
     g.t = zgemm_cfg(g.cfg);
     if (g.t.bm == 0) {
@@ -210,14 +211,14 @@ hsgen_status run_grouped(hsgen_ctx* ctx, int cfg, std::vector<GemmProblem>& prob
     return HSGEN_OK;
 }
 
-// Common tail of create_hs / contract_hs: Z, Zs resident -> tmul -> S, H.
+// Common tail of create_hs / contract_hs: Y (or Z, Zs) resident -> tmul -> S, H.
 hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng, const hsgen_opts& o,
                        double* H, double* S, int64_t ld, hsgen_report* rep, double ms_front)
 {
     const int flags = o.flags;
     hsgen_status st;
     double2* Z = (double2*)ctx->Z.p;
-    double2* Zs = (double2*)ctx->Zs.p;
+    double2* Zs = zs_ptr(ctx, geo);
     const bool need_h = !(flags & HSGEN_SKIP_H);
     double2* W = nullptr;
     if (need_h) {
@@ -348,7 +349,8 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
     const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
     int max_ng = 1;
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
-    const Geometry geo = geometry(o.tile, s->kdim, max_ng);
+    Geometry geo = geometry(o.tile, s->kdim, max_ng);
+    geo.fold = s->fold;
     const int64_t tm = std::max(geo.t.bm, geo.t.bn);
     const int tc = tapp_cfg(geo);
     hsgen_status st = ensure_M(ctx, s, geo.t.bk, geo.t.bm);
@@ -363,11 +365,13 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
         hdev[i] = !do_h || is_device_ptr(H[i]);
         sdev[i] = !do_s || is_device_ptr(S[i]);
     }
-    // batches: operands (Z, Zs, W) plus device copies of host outputs within
-    // ~70% of the memory the call may use
+    // batches: operands (Y, W or Z, Zs, W) plus device copies of host outputs within
+    // 90% of the memory the call may use (the rest of the call's device memory,
+    // the graph's problem and tile lists, is a few MB)
+    if (geo.fold) release(ctx->Zs);   // folded operands need no Zs
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    size_t budget = (size_t)(0.7 * (double)(fr + ctx->Z.bytes + ctx->Zs.bytes + ctx->W.bytes + ctx->btmp.bytes));
+    size_t budget = (size_t)(0.9 * (double)(fr + ctx->Z.bytes + ctx->Zs.bytes + ctx->W.bytes + ctx->btmp.bytes));
     if (const char* e = std::getenv("HSGEN_BATCH_BYTES"))   // cap (tuning and tests)
         if (std::strtoull(e, nullptr, 10) > 0) budget = std::min<size_t>(budget, std::strtoull(e, nullptr, 10));
     std::vector<std::vector<int>> batches;
@@ -376,7 +380,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
         std::vector<int> cur;
         for (int i : order) {
             const size_t gp = (size_t)roundup(std::max(kps[i].ng, 1), tm);
-            const size_t need = 3 * sizeof(double2) * (size_t)geo.ldz * gp +
+            const size_t need = n_operands(geo) * sizeof(double2) * (size_t)geo.ldz * gp +
                                 sizeof(double2) * (size_t)kps[i].ng * kps[i].ng * ((!hdev[i]) + (!sdev[i]));
             if (!cur.empty() && used + need > budget) {
                 batches.push_back(cur);
@@ -405,8 +409,10 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
         const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * ncols;
         st = ensure(ctx, ctx->Z, zbytes, false);
         if (st) return st;
-        st = ensure(ctx, ctx->Zs, zbytes, false);
-        if (st) return st;
+        if (!geo.fold) {
+            st = ensure(ctx, ctx->Zs, zbytes, false);
+            if (st) return st;
+        }
         if (do_h) {
             st = ensure(ctx, ctx->W, zbytes, true);
             if (st) return st;
@@ -416,7 +422,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             if (st) return st;
         }
         double2* Z = (double2*)ctx->Z.p;
-        double2* Zs = (double2*)ctx->Zs.p;
+        double2* Zs = zs_ptr(ctx, geo);
         double2* W = (double2*)ctx->W.p;
         // device outputs of every k of the batch (the caller's, or scratch)
         std::vector<double2*> dH(ks.size(), nullptr), dS(ks.size(), nullptr);
@@ -500,8 +506,9 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
                 AbgenParams p = abgen_params(s, &kps[i], (const int32_t*)bg.g.p + goff[q]);
                 p.ncols = (int32_t)roundup(std::max(kps[i].ng, 1), tm);
                 p.Z = Z + col0[q] * geo.ldz;
-                p.Zs = Zs + col0[q] * geo.ldz;
+                p.Zs = geo.fold ? nullptr : Zs + col0[q] * geo.ldz;
                 p.ldz = geo.ldz;
+                p.fold = geo.fold;
                 ce = launch_abgen(p, true, ctx->stream);
             }
             const GemmProblem* dp = (const GemmProblem*)bg.probs.p;
@@ -559,7 +566,7 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             r.ng = kps[i].ng;
             r.kdim = s->kdim;
             r.flops_alg = 16.0 * (double)kps[i].ng * kps[i].ng * s->kdim + 32.0 * (double)kps[i].ng * (double)s->sum_nlm2;
-            r.bytes_abgen = 2.0 * 2.0 * 16.0 * (double)kps[i].ng * s->kdim;
+            r.bytes_abgen = (geo.fold ? 1.0 : 2.0) * 2.0 * 16.0 * (double)kps[i].ng * s->kdim;
             r.tile_used = geo.cfg;
         }
         if (nk > 0) rep[0].ms_total = ms_all;
@@ -632,20 +639,24 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (rep) std::memset(rep, 0, sizeof *rep);
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
-                                  o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
+    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
+                            o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
+    geo.fold = s->fold;
     st = upload_g(ctx, kp);
     if (st) return st;
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
     st = ensure(ctx, ctx->Z, zbytes, false);
     if (st) return st;
-    st = ensure(ctx, ctx->Zs, zbytes, false);
-    if (st) return st;
+    if (!geo.fold) {
+        st = ensure(ctx, ctx->Zs, zbytes, false);
+        if (st) return st;
+    }
     AbgenParams p = abgen_params(s, kp, (const int32_t*)ctx->g.p);
     p.ncols = (int32_t)geo.gp;
     p.Z = (double2*)ctx->Z.p;
-    p.Zs = (double2*)ctx->Zs.p;
+    p.Zs = geo.fold ? nullptr : (double2*)ctx->Zs.p;
     p.ldz = geo.ldz;
+    p.fold = geo.fold;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_abgen(p, true, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -656,7 +667,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
         HIPCHK(hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[1]));
         rep->ms_abgen = t0;
         rep->ms_total += t0;
-        rep->bytes_abgen = 2.0 * 2.0 * 16.0 * (double)ng * s->kdim;   // Z and Zs, A and B rows
+        rep->bytes_abgen = (geo.fold ? 1.0 : 2.0) * 2.0 * 16.0 * (double)ng * s->kdim;   // Y, or Z and Zs
     }
     return HSGEN_OK;
 }
@@ -727,7 +738,11 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     s.L = L;
     int off = 0;
     s.atoms.resize(a);
+    // B-row scales: sqrt_u_norms themselves (folded operand Y = [A; diag(sqrt_u_norms) B],
+    // as the reference's W, h_and_s_stripped_v2.cpp:64-86) or their squares (Zs)
     std::vector<double> un2((size_t)a * s.nl, 0.0);
+    s.h_sqrtn.assign((size_t)a * s.nl, 0.0);
+    bool fold = fold_enabled();
     for (int i = 0; i < a; ++i) {
         const int lm = l_maxs[i];
         if (lm < 0 || lm > max_lmax) return fail(ctx, HSGEN_ERR_ARG, "l_maxs[%d] out of range", i);
@@ -744,14 +759,19 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
         for (int l = 0; l <= lm; ++l) {
             const double v = sqrt_u_norms[(size_t)i * s.nl + l];
             un2[(size_t)i * s.nl + l] = v * v;
+            s.h_sqrtn[(size_t)i * s.nl + l] = v;
+            if (!fold_ok(v * v)) fold = false;
         }
     }
     s.kdim = off;
+    s.fold = fold;
     if (G == 0) return HSGEN_OK;
     o.tile = pick_tile(ctx, o, G, s.kdim);
-    const Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
-                                  o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
-    hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, geo.t.bk, geo.t.bm);
+    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
+                            o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
+    geo.fold = s.fold;
+    hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, s.fold ? s.h_sqrtn.data() : nullptr, geo.t.bk,
+                               geo.t.bm);
     if (st) return st;
     struct Guard {
         hsgen_sys* s;
@@ -779,18 +799,20 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     if (st) return st;
     HIPCHK(hipMemcpyAsync(ctx->atoms_tmp.p, s.atoms.data(), sizeof(AtomDev) * a, hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->unorm2.p, un2.data(), sizeof(double) * un2.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->unorm2.p, s.fold ? s.h_sqrtn.data() : un2.data(), sizeof(double) * un2.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * geo.gp;
     st = ensure(ctx, ctx->Z, zbytes, false);
     if (st) return st;
-    st = ensure(ctx, ctx->Zs, zbytes, false);
-    if (st) return st;
+    if (!geo.fold) {
+        st = ensure(ctx, ctx->Zs, zbytes, false);
+        if (st) return st;
+    }
     PackParams pp;
     pp.A = dA;
     pp.B = dB;
     pp.atoms = (const AtomDev*)ctx->atoms_tmp.p;
-    pp.unorm2 = (const double*)ctx->unorm2.p;
+    pp.scale = (const double*)ctx->unorm2.p;
     pp.natoms = a;
     pp.L = L;
     pp.nl = s.nl;
@@ -798,8 +820,9 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     pp.kdim = s.kdim;
     pp.ldab = (int64_t)L * a;
     pp.Z = (double2*)ctx->Z.p;
-    pp.Zs = (double2*)ctx->Zs.p;
+    pp.Zs = geo.fold ? nullptr : (double2*)ctx->Zs.p;
     pp.ldz = geo.ldz;
+    pp.fold = geo.fold;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_pack_ref(pp, (int)geo.gp, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));

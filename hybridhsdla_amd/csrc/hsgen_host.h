@@ -7,11 +7,15 @@
 //   tune_eig.hip    the tile autotuner and the generalized eigensolve
 //   noalloc.hip     the allocation-free FP64 entry point
 //
-// Operand layout in HBM, per k-point (complex128, column-major, ld = ldz):
-//   Z  = [A_1; B_1; A_2; B_2; ...]   rows 2*off_a .. 2*off_a + 2*nlm_a - 1
-//   Zs = [A_1; N_1 B_1; ...]         N_a = diag(||u_dot_l||^2)
-//   W  = [M_1 Z_1; M_2 Z_2; ...]     M_a = [[T_aa, T_ab], [T_ab^H, T_bb]]
-// so that   S = Z^H Zs   and   H = Z^H W   (only i >= j stored).
+// Operand layout in HBM, per k-point (complex128, column-major, ld = ldz).
+// Folded (every ||u_dot_l||^2 > 0, the usual case):
+//   Y  = [A_1; N_1^1/2 B_1; A_2; ...]   rows 2*off_a .. 2*off_a + 2*nlm_a - 1
+//   W  = [M'_1 Y_1; M'_2 Y_2; ...]      M'_a = D_a^-1/2 M_a D_a^-1/2
+// so that   S = Y^H Y   and   H = Y^H W   (only i >= j stored); otherwise
+//   Z  = [A_1; B_1; A_2; B_2; ...]
+//   Zs = [A_1; N_1 B_1; ...]            N_a = diag(||u_dot_l||^2)
+//   W  = [M_1 Z_1; M_2 Z_2; ...]        M_a = [[T_aa, T_ab], [T_ab^H, T_bb]]
+// so that   S = Z^H Zs   and   H = Z^H W.  Y lives where Z does (ctx->Z).
 // ldz = roundup(2K, BK) + BK, columns padded to the tile size; every padding
 // element is zero so the kernels need no bounds checks along k.
 #pragma once
@@ -89,6 +93,11 @@ struct hsgen_sys {
     std::vector<int32_t> m_cols;       // per atom: padded column count
     int m_bk = 0, m_bm = 0;            // tile geometry M was padded for
     int64_t sum_nlm2 = 0;              // sum_a nlm_a^2
+    // folded operands: Y = [A_a; sqrt(N_a) B_a] per atom (S = Y^H Y,
+    // H = Y^H M' Y) when every N_l = ||u_dot_l||^2 is positive; otherwise
+    // Z and Zs = [A_a; N_a B_a] (S = Z^H Zs, H = Z^H M Z)
+    bool fold = false;
+    std::vector<double> h_sqrtn;       // [natoms][nl] sqrt(N_l), 0 for l > lmax
     hsgen::AtomDev* d_atoms = nullptr;
     hsgen::RadDev* d_rad = nullptr;
     double* d_ynorm = nullptr;
@@ -125,7 +134,11 @@ bool is_device_ptr(const void* p);
 // M_a blocks of every atom padded for tile geometry (bk, bm), deduplicated by
 // T pointer triple, uploaded to s->d_M.
 hsgen_status upload_M(hsgen_ctx* ctx, hsgen_sys* s, const int32_t* nlm, const double* const* taa,
-                      const double* const* tab, const double* const* tbb, int bk, int bm);
+                      const double* const* tab, const double* const* tbb, const double* rsc, int bk, int bm);
+// Folded operands unless HSGEN_FOLD=0 (environment; tests and comparisons).
+bool fold_enabled();
+// N_l usable for folding (positive, 1/sqrt(N) far from overflow).
+bool fold_ok(double n);
 hsgen_status ensure_M(hsgen_ctx* ctx, hsgen_sys* s, int bk, int bm);
 hsgen_status check_kpoint(hsgen_ctx* ctx, const hsgen_kpoint* kp);
 hsgen_status upload_g(hsgen_ctx* ctx, const hsgen_kpoint* kp);
@@ -138,7 +151,13 @@ struct Geometry {
     int64_t kpad;   // kdepth of the S/H contraction = roundup(2K, BK)
     int64_t ldz;    // kpad + BK
     int64_t gp;     // padded column count
+    bool fold;      // operands Y (in Z) and W: S = Y^H Y; else Z, Zs, W
 };
+
+// FP64 operand arrays per column set: Y, W (folded) or Z, Zs, W.
+inline int n_operands(const Geometry& g) { return g.fold ? 2 : 3; }
+// The right operand of S: Zs, or Y itself (in ctx->Z) when folded.
+inline double2* zs_ptr(hsgen_ctx* ctx, const Geometry& g) { return (double2*)(g.fold ? ctx->Z.p : ctx->Zs.p); }
 
 int64_t int8_col_align(int tile);
 Geometry geometry(int tile, int kdim, int ng, int64_t col_align = 1);

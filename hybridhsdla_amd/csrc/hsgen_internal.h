@@ -28,6 +28,7 @@ struct RadDev {
     double u, u_dr, dE_u, dE_u_dr;
     double inv_wronk;
     double dE_u_norm;
+    double sqrt_un;    // sqrt(dE_u_norm), taken on the host (h_and_s.cpp:710)
 };
 
 // One problem of the grouped complex GEMM  C = L^H R  (complex128).
@@ -68,28 +69,32 @@ struct AbgenParams {
     int32_t nrot, nrmt;
     int32_t rot_rep[kAbgenMaxCls];
     int32_t rmt_rep[kAbgenMaxCls];
-    // compact
+    // compact: Z = [A_a; B_a] and Zs = [A_a; N_a B_a] per atom, or, with
+    // fold, only Z = [A_a; sqrt(N_a) B_a] (Zs unused)
     double2* Z;
     double2* Zs;
     int64_t ldz;
+    int32_t fold;
     // ref
     double2* A;
     double2* B;
 };
 
 // Pack reference-layout A, B (L*a x G, ld = L*a) into the compact operands
-// Z = [A_a; B_a] per atom and Zs = [A_a; N_a B_a], N = sqrt_u_norms^2
+// Z = [A_a; B_a] per atom and Zs = [A_a; N_a B_a], N = sqrt_u_norms^2, or,
+// with fold, into Z = [A_a; sqrt_u_norms B_a] alone
 // (h_and_s_stripped_v2.cpp:64-86 applies sqrt_u_norms to B before the herk).
 struct PackParams {
     const double2* A;
     const double2* B;
     const AtomDev* atoms;
-    const double* unorm2;   // [natoms][nl] squared sqrt_u_norms
+    const double* scale;    // [natoms][nl] B-row scale: sqrt_u_norms (fold) or their squares
     int32_t natoms, L, nl, ng, kdim;
     int64_t ldab;           // L * natoms
     double2* Z;
     double2* Zs;
     int64_t ldz;
+    int32_t fold;
 };
 
 hipError_t launch_abgen(const AbgenParams& p, bool compact, hipStream_t st);
@@ -120,7 +125,7 @@ struct OzResParams {
     int32_t nmod;
     int32_t p[kMaxModuli];
     double invp[kMaxModuli];
-    int8_t* out[kMaxModuli];     // (re, im) residue pairs, column stride ldo bytes
+    int8_t* out[kMaxModuli];     // (re, im) residue pairs, column stride ldo bytes, or null
     int8_t* out_rot[kMaxModuli]; // rotated pairs or null
     int32_t rot_kind;            // 1: (-im, re) (left operand), 2: (im, -re) (right operand)
     int64_t ldo;

@@ -3,7 +3,8 @@
 // serving loops, harnesses that forbid it).
 //
 // Every workspace of the call (the G list when it is host memory, the grouped
-// GEMM problem and tile lists, the operands Z, Zs and W) lives in `scratch`,
+// GEMM problem and tile lists, the operands Y and W, or Z, Zs and W for a
+// system that does not fold ||u_dot||^2 into Y) lives in `scratch`,
 // one file-scope device array of HSGEN_NOALLOC_SCRATCH_BYTES that each GPU the
 // library runs on holds until the process ends.  In host code the array's name
 // is only a host-side stand-in: the address of each GPU's copy comes from
@@ -14,7 +15,7 @@
 // The tile lists are written on the GPU by k_noalloc_plan, so the call copies
 // nothing from host memory except a host G list.  Every byte of scratch that
 // a kernel reads is written earlier in the same call (the generator writes
-// every row and padding column of Z and Zs, the T application every column of
+// every row and padding column of Y (or Z and Zs), the T application every column of
 // W and k_clear_rows its rows past 2K), so nothing needs clearing between
 // calls.  Calls on one GPU share its array and the last call's contents stay
 // there: calls that overlap on one GPU, from any contexts, race.
@@ -187,7 +188,8 @@ extern "C" hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys*
     char* base = (char*)scratch_of(ctx->device);
     if (!base) return fail(ctx, HSGEN_ERR_UNSUPPORTED, "no scratch array on device %d", ctx->device);
     o.tile = pick_tile(ctx, o, ng, s->kdim);
-    const Geometry geo = geometry(o.tile, s->kdim, ng);
+    Geometry geo = geometry(o.tile, s->kdim, ng);
+    geo.fold = s->fold;
     const int tc = tapp_cfg(geo);
     const ZgemmLaunchCfg tcc = zgemm_cfg(tc);
     if (!s->d_M || !s->d_mloc || s->m_bk != geo.t.bk || s->m_bm != geo.t.bm)
@@ -219,7 +221,7 @@ extern "C" hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys*
     const size_t o_pre = take(sizeof(int32_t) * (size_t)(na + 1));
     const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * (size_t)geo.gp;
     const size_t o_z = take(zbytes);
-    const size_t o_zs = take(zbytes);
+    const size_t o_zs = geo.fold ? o_z : take(zbytes);
     const size_t o_w = do_h ? take(zbytes) : 0;
     if (off > kScratchBytes)
         return fail(ctx, HSGEN_ERR_ARG,
@@ -269,8 +271,9 @@ extern "C" hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys*
     AbgenParams p = abgen_params(s, kp, dg);
     p.ncols = (int32_t)geo.gp;
     p.Z = Z;
-    p.Zs = Zs;
+    p.Zs = geo.fold ? nullptr : Zs;
     p.ldz = geo.ldz;
+    p.fold = geo.fold;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_abgen(p, true, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -298,7 +301,7 @@ extern "C" hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys*
         rep->ng = ng;
         rep->kdim = s->kdim;
         rep->flops_alg = 16.0 * (double)ng * ng * s->kdim + 32.0 * (double)ng * (double)s->sum_nlm2;
-        rep->bytes_abgen = 2.0 * 2.0 * 16.0 * (double)ng * s->kdim;
+        rep->bytes_abgen = (geo.fold ? 1.0 : 2.0) * 2.0 * 16.0 * (double)ng * s->kdim;
         rep->tile_used = geo.cfg;
     }
     return HSGEN_OK;

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_residues(OzResParams p)
             const int b = p.rot_kind == 1 ? rr : -rr;
             wr[k >> 1] |= ((uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8)) << s;
         }
-        *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(w[0], w[1]);
+        if (p.out[m]) *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(w[0], w[1]);
         if (p.out_rot[m]) *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(wr[0], wr[1]);
     }
 }

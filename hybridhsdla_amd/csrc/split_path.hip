@@ -113,6 +113,27 @@ double oz_model(const int32_t* el, const double* sl, const int32_t* er, const do
     return tril_norm2 > 0.0 ? std::sqrt((a * b + c * d) / 12.0 / tril_norm2) : 0.0;
 }
 
+// Residue arrays of one modulus ("slot") for INT8 configuration ic: those of
+// Z (or Y), of Zs (none when folded: S reads Y's for both operands), of W,
+// and the rotated copies the kernel reads; indices into the slot, -1 absent.
+struct OzSlotLayout {
+    int z, zs, w, zrot, zsrot, wrot, narr;
+};
+
+OzSlotLayout oz_slots(const IgemmLaunchCfg& ic, bool fold)
+{
+    OzSlotLayout l;
+    int n = 0;
+    l.z = n++;
+    l.zs = fold ? l.z : n++;
+    l.w = n++;
+    l.zrot = ic.needs_rot == 1 ? n++ : -1;    // (-Li, Lr) of the left operand
+    l.zsrot = ic.needs_rot == 2 ? n++ : -1;   // (Ri, -Rr) of S's right operand
+    l.wrot = ic.needs_rot == 2 ? n++ : -1;    // (Ri, -Rr) of W
+    l.narr = n;
+    return l;
+}
+
 // One split-precision evaluation of every k-point with nmod moduli: residues,
 // INT8 GEMMs that store C mod p (one launch per modulus over the tiles of all
 // k-points), final recombination into the outputs.
@@ -125,7 +146,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
     const OzPlan pl = oz_plan(nmod);
     const double2* Z = (const double2*)ctx->Z.p;
-    const double2* Zs = (const double2*)ctx->Zs.p;
+    const double2* Zs = zs_ptr(ctx, geo);
     const double2* W = (const double2*)ctx->W.p;
     // tile list (shared by every modulus; per k-point the FP64 path's order,
     // problem ids S (if any) then H of each k-point in turn); a tile's slot is
@@ -148,8 +169,8 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     hsgen_status st = ensure(ctx, ctx->oz_out, std::max<size_t>(mod_stride * nmod, 1), false);
     if (st) return st;
     uint8_t* rout = (uint8_t*)ctx->oz_out.p;
-    // Residue arrays of one modulus ("slot"): Zr, Zs_r, W_r (+ rotated copies
-    // the kernel reads), 2 bytes per complex.  As many moduli as fit in 60% of
+    // Residue arrays of one modulus ("slot"): Zr, Zs_r, W_r (no Zs_r when
+    // folded) and rotated copies the kernel reads, 2 bytes per complex.  As many moduli as fit in 60% of
     // the free memory share one residue pass, which reads the FP64 operands
     // once for the whole group.
     const int64_t ldo = 2 * kd;
@@ -157,8 +178,8 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
         return fail(ctx, HSGEN_ERR_UNSUPPORTED, "INT8 contraction depth %lld too large for 32-bit tile offsets",
                     (long long)kd);
     const size_t arr = (size_t)ldo * gp;
-    const int narr = ic.needs_rot == 1 ? 4 : ic.needs_rot == 2 ? 5 : 3;
-    const size_t slot_bytes = narr * arr;
+    const OzSlotLayout sl = oz_slots(ic, geo.fold);
+    const size_t slot_bytes = sl.narr * arr;
     int group = 1;
     {
         size_t fr = 0, tot = 0;
@@ -180,13 +201,14 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     const int64_t kc = ic.max_kchunk > 0 ? ic.max_kchunk : kd;
     const int nchunk = (int)((kd + kc - 1) / kc);
     size_t nprob = 0;
+    auto slot_or_null = [&](int i, int a) -> int8_t* { return a >= 0 ? slot(i, a) : nullptr; };
     for (int i = 0; i < group; ++i) {
-        int8_t* Zr = slot(i, 0);
-        int8_t* Zsr = slot(i, 1);
-        int8_t* Wr = slot(i, 2);
-        int8_t* Zrot = ic.needs_rot == 1 ? slot(i, 3) : nullptr;     // (-Li, Lr) of Z
-        int8_t* Zsrot = ic.needs_rot == 2 ? slot(i, 3) : nullptr;    // (Ri, -Rr) of Zs
-        int8_t* Wrot = ic.needs_rot == 2 ? slot(i, 4) : nullptr;     // (Ri, -Rr) of W
+        int8_t* Zr = slot(i, sl.z);
+        int8_t* Zsr = slot(i, sl.zs);
+        int8_t* Wr = slot(i, sl.w);
+        int8_t* Zrot = slot_or_null(i, sl.zrot);
+        int8_t* Zsrot = slot_or_null(i, sl.zsrot);
+        int8_t* Wrot = slot_or_null(i, sl.wrot);
         std::vector<IgemmProblem> base;
         for (const OzK& k : ks) {
             const int64_t o = k.col0 * ldo;                           // bytes
@@ -263,7 +285,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
                 rp.X = X + c0 * geo.ldz;
                 rp.e = e + c0;
                 for (int i = 0; i < gn; ++i) {
-                    rp.out[i] = slot(i, a) + c0 * ldo;
+                    rp.out[i] = a >= 0 ? slot(i, a) + c0 * ldo : nullptr;
                     rp.out_rot[i] = arot >= 0 ? slot(i, arot) + c0 * ldo : nullptr;
                 }
                 const hipError_t err = launch_residues(rp, (int)(rg.second - c0), ctx->stream);
@@ -272,10 +294,12 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
             return hipSuccess;
         };
         rp.rot_kind = 1;
-        HIPCHK(residues(Z, eZ, 0, ic.needs_rot == 1 ? 3 : -1));
+        HIPCHK(residues(Z, eZ, sl.z, sl.zrot));
         rp.rot_kind = 2;
-        if (do_s) HIPCHK(residues(Zs, eZs, 1, ic.needs_rot == 2 ? 3 : -1));
-        if (do_h) HIPCHK(residues(W, eW, 2, ic.needs_rot == 2 ? 4 : -1));
+        // folded: S's right operand is Y, whose residues the left pass wrote;
+        // only a rotated copy, if the kernel reads one, is still missing
+        if (do_s && (!geo.fold || sl.zsrot >= 0)) HIPCHK(residues(Zs, eZs, geo.fold ? -1 : sl.zs, sl.zsrot));
+        if (do_h) HIPCHK(residues(W, eW, sl.w, sl.wrot));
         HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups], ctx->stream));
         for (int i = 0; i < gn; ++i) {
             const int m = m0 + i;
@@ -388,10 +412,16 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
     if (nbt > 0)
         HIPCHK(hipMemcpyAsync(dcols, cols.data(), sizeof(int32_t) * nbt, hipMemcpyHostToDevice, ctx->stream));
     const double2* Z = (const double2*)ctx->Z.p;
-    const double2* Zs = (const double2*)ctx->Zs.p;
+    const double2* Zs = zs_ptr(ctx, geo);
     const double2* W = (const double2*)ctx->W.p;
     HIPCHK(launch_colexp(Z, geo.ldz, (int)geo.ldz, (int)gp, eZ, ssZ, ctx->stream));
-    if (do_s) HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
+    if (do_s && geo.fold) {
+        // S's right operand is Y itself: the same exponents and norms
+        HIPCHK(hipMemcpyAsync(eZs, eZ, sizeof(int32_t) * gp, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ssZs, ssZ, sizeof(double) * gp, hipMemcpyDeviceToDevice, ctx->stream));
+    } else if (do_s) {
+        HIPCHK(launch_colexp(Zs, geo.ldz, (int)geo.ldz, (int)gp, eZs, ssZs, ctx->stream));
+    }
     if (do_h) HIPCHK(launch_colexp(W, geo.ldz, (int)geo.ldz, (int)gp, eW, ssW, ctx->stream));
     // FP64 samples: columns cols[] of S and H of each k-point, all rows, ld = ng
     // (the right operand's sampled columns are gathered first: the GEMM reads
@@ -611,10 +641,11 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
     int max_ng = 1;
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
     const int64_t align = int8_col_align(icfg);
-    const Geometry g0 = geometry(0, s->kdim, max_ng, align);
+    Geometry g0 = geometry(0, s->kdim, max_ng, align);
+    g0.fold = s->fold;
     const int64_t tm = std::max<int64_t>({(int64_t)g0.t.bm, (int64_t)g0.t.bn, align});
     const int64_t kd = roundup(g0.kpad, ic.bk);
-    const int narr = ic.needs_rot == 1 ? 4 : ic.needs_rot == 2 ? 5 : 3;
+    const int narr = oz_slots(ic, g0.fold).narr;
     hsgen_status st = ensure_M(ctx, s, g0.t.bk, g0.t.bm);
     if (st) return st;
     if (rep) std::memset(rep, 0, sizeof(hsgen_report) * nk);
@@ -636,6 +667,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
     // (HSGEN_RESIDUE_PASSES for tuning; each pass reads Z, Zs, W once, and
     // one pass per batch saves those reads but needs more, smaller batches,
     // which cost as much again at 64 k-points)
+    if (g0.fold) release(ctx->Zs);   // folded operands need no Zs
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     size_t held = 0;
@@ -660,7 +692,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
         size_t tiles = 0;
         for (size_t bi = 0; bi < ntr; ++bi)
             tiles += std::min(ntc, ((bi + 1) * ic.bm + ic.bn - 1) / ic.bn);
-        need[i] = 3 * sizeof(double2) * (size_t)g0.ldz * gp + sizeof(double2) * ng * ng * ((!hdev[i]) + (!sdev[i])) +
+        need[i] = n_operands(g0) * sizeof(double2) * (size_t)g0.ldz * gp + sizeof(double2) * ng * ng * ((!hdev[i]) + (!sdev[i])) +
                   (size_t)(do_s + do_h) * tiles * 2 * ic.bm * ic.bn * nmod_est + (size_t)gslots * narr * 2 * kd * gp;
         total += need[i];
     }
@@ -707,8 +739,10 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
         const size_t zbytes = sizeof(double2) * (size_t)geo.ldz * ncols;
         st = ensure(ctx, ctx->Z, zbytes, false);
         if (st) return st;
-        st = ensure(ctx, ctx->Zs, zbytes, false);
-        if (st) return st;
+        if (!geo.fold) {
+            st = ensure(ctx, ctx->Zs, zbytes, false);
+            if (st) return st;
+        }
         if (do_h) {
             st = ensure(ctx, ctx->W, zbytes, true);
             if (st) return st;
@@ -718,7 +752,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
             if (st) return st;
         }
         double2* Z = (double2*)ctx->Z.p;
-        double2* Zs = (double2*)ctx->Zs.p;
+        double2* Zs = zs_ptr(ctx, geo);
         double2* W = (double2*)ctx->W.p;
         // G lists, concatenated in batch order
         std::vector<int32_t> gall;
@@ -753,8 +787,9 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
             AbgenParams p = abgen_params(s, &kps[i], (const int32_t*)ctx->g.p + goff[q]);
             p.ncols = (int32_t)ks[q].ncols;
             p.Z = Z + col0[q] * geo.ldz;
-            p.Zs = Zs + col0[q] * geo.ldz;
+            p.Zs = geo.fold ? nullptr : Zs + col0[q] * geo.ldz;
             p.ldz = geo.ldz;
+            p.fold = geo.fold;
             HIPCHK(launch_abgen(p, true, ctx->stream));
         }
         if (do_h) {
@@ -807,7 +842,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
             r.ng = kps[i].ng;
             r.kdim = s->kdim;
             r.flops_alg = 16.0 * (double)kps[i].ng * kps[i].ng * s->kdim + 32.0 * (double)kps[i].ng * (double)s->sum_nlm2;
-            r.bytes_abgen = 2.0 * 2.0 * 16.0 * (double)kps[i].ng * s->kdim;
+            r.bytes_abgen = (g0.fold ? 1.0 : 2.0) * 2.0 * 16.0 * (double)kps[i].ng * s->kdim;
             r.tile_used = icfg;
         }
         if (nk > 0) rep[0].ms_total = ms_all;

@@ -158,6 +158,55 @@ def test_repeated_g_vectors(ctx, big_list):
             assert np.abs(M[:, a] - M[:, b]).max() <= 1e-14 * np.abs(M).max()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_unfolded_operands_when_a_norm_is_zero(ctx, precision):
+    """A zero ||u_dot_l||^2 leaves B out of S for that l; the operand cannot
+    carry sqrt(N) B then, so the library keeps Z, Zs = [A; N B] and
+    W = M Z (S = Z^H Zs, H = Z^H W).  Against the oracle."""
+    from hybridhsdla_amd.api import HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=3, lmax=(3, 2, 4), ng_target=300, seed=9)
+    s.match[0, 2, 4] = 0.0
+    h = ctx.upload(s)
+    prec, tol = (HSGEN_PREC_FP64, TOL) if precision == "fp64" else (HSGEN_PREC_OZAKI_INT8, 1e-12)
+    H, S = ctx.create_HS2(h, kp, precision=prec, target_rel_err=1e-12)
+    Hr, Sr = po.reference_hs(s, kp)
+    eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
+    print(f"{precision}: H {eh:.2e} S {es:.2e}", flush=True)
+    assert eh <= tol and es <= tol
+    un = s.sqrt_u_norms()
+    Hc, Sc = ctx.create_HS2_stripped_v2(s.L, s.natoms, kp.ng, *po.construct_ab(s, kp), s.taa, s.tab, s.tbb, s.lmax,
+                                        s.max_lmax, un, precision=prec, target_rel_err=1e-12)
+    assert rel_fro_lower(Hc, Hr) <= tol and rel_fro_lower(Sc, Sr) <= tol
+
+
+def test_folded_and_unfolded_operands_agree(ctx, monkeypatch, big_list):
+    """HSGEN_FOLD=0 keeps the three-array form for a system whose norms all
+    allow folding: both forms agree with the oracle and with each other to
+    FP64 rounding, in FP64 and in split precision, through create_hs and
+    through the contraction-only entry point."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = big_list
+    sub = _subset(kp, np.arange(300) * 2)
+    Hr, Sr = po.reference_hs(s, sub)
+    A, B = po.construct_ab(s, sub)
+    out = {}
+    for fold in ("1", "0"):
+        monkeypatch.setenv("HSGEN_FOLD", fold)
+        h = ctx.upload(s)
+        H, S = ctx.create_HS2(h, sub)
+        Hs, Ss, rep = ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+        Hc, Sc = ctx.create_HS2_stripped_v2(s.L, s.natoms, sub.ng, A, B, s.taa, s.tab, s.tbb, s.lmax, s.max_lmax,
+                                            s.sqrt_u_norms())
+        h.close()
+        print(f"fold={fold}: FP64 H {rel_fro_lower(H, Hr):.2e} S {rel_fro_lower(S, Sr):.2e}; split "
+              f"({rep['slices_used']} moduli) H {rel_fro_lower(Hs, Hr):.2e} S {rel_fro_lower(Ss, Sr):.2e}", flush=True)
+        assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
+        assert rel_fro_lower(Hc, Hr) <= TOL and rel_fro_lower(Sc, Sr) <= TOL
+        assert rel_fro_lower(Hs, Hr) <= 1e-12 and rel_fro_lower(Ss, Sr) <= 1e-12
+        out[fold] = (H, S)
+    assert rel_fro_lower(out["1"][0], out["0"][0]) <= 1e-14 and rel_fro_lower(out["1"][1], out["0"][1]) <= 1e-14
+
+
 def test_batched_ragged_sizes_equal_single_calls(ctx, big_list):
     """Host outputs, so that the empty k-point also goes through the
     device-to-host copy of the batched path."""

@@ -162,16 +162,20 @@ def test_split_precision_error_falls_with_moduli(ctx):
     assert errs[-1] <= 1e-14
 
 
-def _rounding_model(s, kp, n):
-    """C' = round(L)^H round(R) for S (R = D Z) and H (R = M Z), with every
-    element of a column rounded to beta bits of the column's 2-norm exponent
-    (norm < 2^e): the exact result the split-precision path must produce
-    when P covers the integer products."""
+def _rounding_model(s, kp, n, fold=True):
+    """C' = round(L)^H round(R), with every element of a column rounded to
+    beta bits of the column's 2-norm exponent (norm < 2^e): the exact result
+    the split-precision path must produce when P covers the integer products.
+    Folded operands (the library's form when every ||u_dot||^2 > 0):
+    L = Y = [A; sqrt(N) B], R = Y for S and R = M' Y for H, with
+    M' = D^-1/2 M D^-1/2 built as the library builds it; otherwise L = Z,
+    R = D Z for S and R = M Z for H."""
     import math
     mods = [255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199]
     beta = min(52, int(math.floor((sum(math.log2(p) for p in mods[:n]) - 1.01) / 2)))
     A, B = po.construct_ab(s, kp)
-    un2 = s.sqrt_u_norms() ** 2
+    sq = s.sqrt_u_norms()
+    un2 = sq ** 2
     lm_l = np.concatenate([[l] * (2 * l + 1) for l in range(s.max_lmax + 1)])
     Z, Zs, W = [], [], []
     for a in range(s.natoms):
@@ -179,9 +183,16 @@ def _rounding_model(s, kp, n):
         Aa, Ba = A[:nlm, a, :], B[:nlm, a, :]
         taa = np.tril(s.taa[a]) + np.tril(s.taa[a], -1).conj().T
         tbb = np.tril(s.tbb[a]) + np.tril(s.tbb[a], -1).conj().T
+        tab, tba = s.tab[a], s.tab[a].conj().T
+        if fold:
+            r = sq[a][lm_l[:nlm]]
+            Ba = r[:, None] * Ba
+            tab, tba, tbb = tab / r[None, :], tba / r[:, None], tbb / (r[:, None] * r[None, :])
+            Zs += [Aa, Ba]
+        else:
+            Zs += [Aa, un2[a][lm_l[:nlm]][:, None] * Ba]
         Z += [Aa, Ba]
-        Zs += [Aa, un2[a][lm_l[:nlm]][:, None] * Ba]
-        W += [taa @ Aa + s.tab[a] @ Ba, s.tab[a].conj().T @ Aa + tbb @ Ba]
+        W += [taa @ Aa + tab @ Ba, tba @ Aa + tbb @ Ba]
     Z, Zs, W = np.vstack(Z), np.vstack(Zs), np.vstack(W)
 
     def rnd(X):
@@ -197,21 +208,24 @@ def _rounding_model(s, kp, n):
     return Hm, Sm
 
 
-def test_split_precision_equals_rounding_model(ctx):
+@pytest.mark.parametrize("fold", [True, False])
+def test_split_precision_equals_rounding_model(ctx, monkeypatch, fold):
     """With 8 moduli (beta = 31) the rounding error is ~1e-9; the GPU result
     must be the exact product of the rounded operands, i.e. agree with the
     CPU rounding model far below that (a wrong scale, exponent or CRT term
-    shows up at the 1e-9 level)."""
+    shows up at the 1e-9 level).  Both operand forms (HSGEN_FOLD=0 keeps
+    Z, Zs, W)."""
     from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    monkeypatch.setenv("HSGEN_FOLD", "1" if fold else "0")
     s, kp = tiny_system(natoms=3, lmax=(3, 4, 2), ng_target=150)
     h = ctx.upload(s)
     H, S = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=8)
-    Hm, Sm = _rounding_model(s, kp, 8)
+    Hm, Sm = _rounding_model(s, kp, 8, fold)
     Hr, Sr = po.reference_hs(s, kp)
-    print(f"vs model H {rel_fro_lower(H, Hm):.3e} S {rel_fro_lower(S, Sm):.3e}; "
+    print(f"fold={fold}: vs model H {rel_fro_lower(H, Hm):.3e} S {rel_fro_lower(S, Sm):.3e}; "
           f"vs FP64 H {rel_fro_lower(H, Hr):.3e} S {rel_fro_lower(S, Sr):.3e}", flush=True)
     assert rel_fro_lower(H, Hr) >= 1e-11 and rel_fro_lower(S, Sr) >= 1e-11
-    # Zs = D Z has the same bits on both sides; W = M Z is summed in another
+    # S's operands have the same bits on both sides; W is summed in another
     # order on the GPU, which can move an element across a rounding boundary
     # (each such flip costs ~1e-13 here), hence the looser bound for H
     assert rel_fro_lower(S, Sm) <= 1e-13
