@@ -384,6 +384,8 @@ hsgen_status hsgen_release_workspace(hsgen_ctx* ctx)
         release(bg.work);
     }
     ctx->bgraphs.clear();
+    for (auto& og : ctx->ozgraphs) destroy_oz_graph(og);
+    ctx->ozgraphs.clear();
     return HSGEN_OK;
 }
 

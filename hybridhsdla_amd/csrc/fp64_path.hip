@@ -80,7 +80,10 @@ void add_hs_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& wor
     // Logical order: patches of kPatchR x kPatchC tiles, S and H tiles of a
     // patch next to each other (they share the Z row panel).  Tiles exist
     // where the tile's first row index >= its first column index.
-    const int kPatchR = 4, kPatchC = 4;
+    // HSGEN_PATCH (environment, tuning) sets another patch side.
+    const char* pe = std::getenv("HSGEN_PATCH");
+    const int kPatch = pe && std::atoi(pe) > 0 ? std::atoi(pe) : 4;
+    const int kPatchR = kPatch, kPatchC = kPatch;
     for (int pi = 0; pi < ntr; pi += kPatchR)
         for (int pj = 0; pj < ntc; pj += kPatchC)
             for (int which = 0; which < 2; ++which) {
@@ -274,7 +277,7 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         ks[0].dS = dS;
         ks[0].ldS = ldS;
         std::vector<OzResult> ozr;
-        st = run_ozaki(ctx, geo, ks, flags, icfg, o.slices, o.target_rel_err, ozr);
+        st = run_ozaki(ctx, geo, ks, flags, icfg, o.slices, o.target_rel_err, ozr, false);
         if (st) return st;
         oz = ozr[0];
     } else {

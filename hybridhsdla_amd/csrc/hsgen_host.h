@@ -22,6 +22,7 @@
 
 #include <complex>
 #include <cstdarg>
+#include <list>
 #include <map>
 #include <string>
 #include <vector>
@@ -37,6 +38,7 @@ constexpr int kDefaultTile = 11;
 constexpr int kFallbackTile = 8;   // register-staged, 64-bit addressing
 constexpr int kIgemmCfg = 51;      // default INT8 kernel configuration (256 x 192 tiles)
 constexpr int kSampleTile = 12;    // FP64 tile of the split-precision sample: 64 x 64, so 64 columns
+constexpr size_t kMaxOzGraphs = 16; // cached split-precision launch-segment sets per context
 
 struct DevBuf {
     void* p = nullptr;
@@ -72,6 +74,16 @@ struct hsgen_ctx {
     };
     std::vector<BatchGraph> bgraphs;
     hsgen::DevBuf btmp;                              // device H, S of host-output k-points
+    // batched split precision: captured launch segments of one evaluation
+    // (per residue group its residue pass and its INT8 launches, then the
+    // recombination) with their own problem and tile lists, replayed while
+    // their key (modulus count, k-points, outputs, workspace) is unchanged
+    struct OzGraph {
+        std::vector<int64_t> key;
+        hsgen::DevBuf probs, work, fprob;
+        std::vector<hipGraphExec_t> seg;
+    };
+    std::list<OzGraph> ozgraphs;                     // oldest first, at most kMaxOzGraphs
     // autotuned tile configurations: (precision, size buckets) -> config id
     std::map<std::vector<int>, int> tuned;
     // generalized eigensolver
@@ -194,8 +206,11 @@ struct OzK {
     int64_t ldS = 0;
 };
 
+// capture: the INT8 launch sequence of each modulus count runs as cached
+// hipGraphs (the batched path), else eagerly.
 hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<OzK>& ks, int flags, int icfg,
-                       int nmod_req, double target, std::vector<OzResult>& res);
+                       int nmod_req, double target, std::vector<OzResult>& res, bool capture);
+void destroy_oz_graph(hsgen_ctx::OzGraph& g);
 hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoint* kps, const hsgen_opts& o,
                            double* const* H, double* const* S, const int64_t* ld, hsgen_report* rep);
 

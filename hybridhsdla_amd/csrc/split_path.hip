@@ -113,6 +113,21 @@ double oz_model(const int32_t* el, const double* sl, const int32_t* er, const do
     return tril_norm2 > 0.0 ? std::sqrt((a * b + c * d) / 12.0 / tril_norm2) : 0.0;
 }
 
+}  // namespace
+
+void destroy_oz_graph(hsgen_ctx::OzGraph& g)
+{
+    for (hipGraphExec_t ex : g.seg)
+        if (ex) (void)hipGraphExecDestroy(ex);
+    g.seg.clear();
+    g.key.clear();
+    release(g.probs);
+    release(g.work);
+    release(g.fprob);
+}
+
+namespace {
+
 // Residue arrays of one modulus ("slot") for INT8 configuration ic: those of
 // Z (or Y), of Zs (none when folded: S reads Y's for both operands), of W,
 // and the rotated copies the kernel reads; indices into the slot, -1 absent.
@@ -136,9 +151,16 @@ OzSlotLayout oz_slots(const IgemmLaunchCfg& ic, bool fold)
 
 // One split-precision evaluation of every k-point with nmod moduli: residues,
 // INT8 GEMMs that store C mod p (one launch per modulus over the tiles of all
-// k-points), final recombination into the outputs.
+// k-points), final recombination into the outputs.  With capture the launch
+// segments (per residue group: its residue pass, then its INT8 launches; then
+// the recombination) are captured into hipGraphs on the first call with a
+// given key (modulus count, k-points, outputs, workspace addresses) and
+// replayed on later ones, with their problem and tile lists in buffers of
+// their own; the HIP events between the segments time the INT8 launches as
+// in the eager run.
 hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<OzK>& ks, int flags, int icfg,
-                        int nmod, const int32_t* eZ, const int32_t* eZs, const int32_t* eW, double* ms_igemm)
+                        int nmod, const int32_t* eZ, const int32_t* eZs, const int32_t* eW, double* ms_igemm,
+                        bool capture)
 {
     const IgemmLaunchCfg ic = igemm_cfg(icfg);
     const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
@@ -236,21 +258,79 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
         if (do_s) fprobs.push_back(OzFinalProblem{k.dS, k.ldS, eZ + k.col0, eZs + k.col0, k.ng, k.ng});
         if (do_h) fprobs.push_back(OzFinalProblem{k.dH, k.ldH, eZ + k.col0, eW + k.col0, k.ng, k.ng});
     }
-    st = ensure(ctx, ctx->oz_probs, sizeof(IgemmProblem) * std::max<size_t>(probs.size(), 1), false);
-    if (st) return st;
-    st = ensure(ctx, ctx->oz_work, sizeof(TileWork) * std::max<size_t>(work.size(), 1), false);
-    if (st) return st;
-    st = ensure(ctx, ctx->oz_fprob, sizeof(OzFinalProblem) * std::max<size_t>(fprobs.size(), 1), false);
-    if (st) return st;
-    if (!probs.empty())
-        HIPCHK(hipMemcpyAsync(ctx->oz_probs.p, probs.data(), sizeof(IgemmProblem) * probs.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
-    if (!work.empty())
-        HIPCHK(hipMemcpyAsync(ctx->oz_work.p, work.data(), sizeof(TileWork) * work.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
-    if (!fprobs.empty())
-        HIPCHK(hipMemcpyAsync(ctx->oz_fprob.p, fprobs.data(), sizeof(OzFinalProblem) * fprobs.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
+    // the cached segments of this key (capture), else the context's shared lists
+    hsgen_ctx::OzGraph* cg = nullptr;
+    bool fresh = false;
+    if (capture) {
+        std::vector<int64_t> key{icfg, nmod, group, flags, (int64_t)geo.fold, geo.ldz, kd,
+                                 (int64_t)(intptr_t)Z, (int64_t)(intptr_t)Zs, (int64_t)(intptr_t)W,
+                                 (int64_t)(intptr_t)ctx->oz_res.p, (int64_t)(intptr_t)rout, (int64_t)(intptr_t)eZ,
+                                 (int64_t)(intptr_t)eZs, (int64_t)(intptr_t)eW};
+        for (const OzK& k : ks)
+            key.insert(key.end(), {k.col0, (int64_t)k.ng, k.ncols, (int64_t)(intptr_t)k.dH, k.ldH,
+                                   (int64_t)(intptr_t)k.dS, k.ldS});
+        for (auto& e : ctx->ozgraphs)
+            if (e.key == key) cg = &e;
+        if (!cg) {
+            if (ctx->ozgraphs.size() >= kMaxOzGraphs) {
+                destroy_oz_graph(ctx->ozgraphs.front());
+                ctx->ozgraphs.pop_front();
+            }
+            ctx->ozgraphs.emplace_back();
+            cg = &ctx->ozgraphs.back();
+            cg->key = key;
+            fresh = true;
+        }
+    }
+    DevBuf& b_probs = cg ? cg->probs : ctx->oz_probs;
+    DevBuf& b_work = cg ? cg->work : ctx->oz_work;
+    DevBuf& b_fprob = cg ? cg->fprob : ctx->oz_fprob;
+    if (!cg || fresh) {
+        st = ensure(ctx, b_probs, sizeof(IgemmProblem) * std::max<size_t>(probs.size(), 1), false);
+        if (st) return st;
+        st = ensure(ctx, b_work, sizeof(TileWork) * std::max<size_t>(work.size(), 1), false);
+        if (st) return st;
+        st = ensure(ctx, b_fprob, sizeof(OzFinalProblem) * std::max<size_t>(fprobs.size(), 1), false);
+        if (st) return st;
+        if (!probs.empty())
+            HIPCHK(hipMemcpyAsync(b_probs.p, probs.data(), sizeof(IgemmProblem) * probs.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        if (!work.empty())
+            HIPCHK(hipMemcpyAsync(b_work.p, work.data(), sizeof(TileWork) * work.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        if (!fprobs.empty())
+            HIPCHK(hipMemcpyAsync(b_fprob.p, fprobs.data(), sizeof(OzFinalProblem) * fprobs.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    // run one launch segment: eagerly, or as the cached graph (captured now
+    // on the key's first call)
+    size_t segi = 0;
+    auto segment = [&](auto&& body) -> hsgen_status {
+        if (!cg) {
+            HIPCHK(body());
+            return HSGEN_OK;
+        }
+        if (fresh) {
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t ce = body();
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+            hipGraphExec_t ex = nullptr;
+            hipError_t ie = hipSuccess;
+            if (ce == hipSuccess && ee == hipSuccess) ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (ce != hipSuccess || ee != hipSuccess || ie != hipSuccess) {
+                cg->key.clear();   // never matches again
+                return fail(ctx, HSGEN_ERR_HIP, "graph capture of the split-precision launches failed: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee != hipSuccess ? ee : ie));
+            }
+            cg->seg.push_back(ex);
+        }
+        if (segi >= cg->seg.size()) return fail(ctx, HSGEN_ERR_HIP, "cached launch segment %zu missing", segi);
+        HIPCHK(hipGraphLaunch(cg->seg[segi], ctx->stream));
+        ++segi;
+        return HSGEN_OK;
+    };
     const int rows = (int)std::min<int64_t>(geo.ldz, kd);
     // column ranges of the k-points (adjacent ones merged): the residue
     // passes convert only these columns
@@ -293,35 +373,46 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
             }
             return hipSuccess;
         };
-        rp.rot_kind = 1;
-        HIPCHK(residues(Z, eZ, sl.z, sl.zrot));
-        rp.rot_kind = 2;
-        // folded: S's right operand is Y, whose residues the left pass wrote;
-        // only a rotated copy, if the kernel reads one, is still missing
-        if (do_s && (!geo.fold || sl.zsrot >= 0)) HIPCHK(residues(Zs, eZs, geo.fold ? -1 : sl.zs, sl.zsrot));
-        if (do_h) HIPCHK(residues(W, eW, sl.w, sl.wrot));
+        st = segment([&]() -> hipError_t {
+            rp.rot_kind = 1;
+            hipError_t e = residues(Z, eZ, sl.z, sl.zrot);
+            rp.rot_kind = 2;
+            // folded: S's right operand is Y, whose residues the left pass
+            // wrote; only a rotated copy, if the kernel reads one, is missing
+            if (e == hipSuccess && do_s && (!geo.fold || sl.zsrot >= 0))
+                e = residues(Zs, eZs, geo.fold ? -1 : sl.zs, sl.zsrot);
+            if (e == hipSuccess && do_h) e = residues(W, eW, sl.w, sl.wrot);
+            return e;
+        });
+        if (st) return st;
         HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups], ctx->stream));
-        for (int i = 0; i < gn; ++i) {
-            const int m = m0 + i;
-            for (int c = 0; c < nchunk; ++c) {
-                IgemmModulus md;
-                md.p = kModuli[m];
-                md.first_chunk = c == 0;
-                md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
-                md.pad = 0;
-                md.invp = 1.0 / kModuli[m];
-                md.out = rout + (size_t)m * mod_stride;
-                HIPCHK(launch_igemm(icfg, (const IgemmProblem*)ctx->oz_probs.p + (i * nchunk + c) * nprob,
-                                    (const TileWork*)ctx->oz_work.p, (int)work.size(), md, ctx->stream));
+        st = segment([&]() -> hipError_t {
+            for (int i = 0; i < gn; ++i) {
+                const int m = m0 + i;
+                for (int c = 0; c < nchunk; ++c) {
+                    IgemmModulus md;
+                    md.p = kModuli[m];
+                    md.first_chunk = c == 0;
+                    md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
+                    md.pad = 0;
+                    md.invp = 1.0 / kModuli[m];
+                    md.out = rout + (size_t)m * mod_stride;
+                    const hipError_t e =
+                        launch_igemm(icfg, (const IgemmProblem*)b_probs.p + (i * nchunk + c) * nprob,
+                                     (const TileWork*)b_work.p, (int)work.size(), md, ctx->stream);
+                    if (e != hipSuccess) return e;
+                }
             }
-        }
+            return hipSuccess;
+        });
+        if (st) return st;
         HIPCHK(hipEventRecord(ctx->oz_ev[2 * ngroups + 1], ctx->stream));
     }
     OzFinalParams fp;
     std::memset(&fp, 0, sizeof fp);
     fp.res = rout;
     fp.mod_stride = (int64_t)mod_stride;
-    fp.prob = (const OzFinalProblem*)ctx->oz_fprob.p;
+    fp.prob = (const OzFinalProblem*)b_fprob.p;
     fp.nmod = nmod;
     fp.beta = pl.beta;
     fp.bm = ic.bm;
@@ -334,7 +425,8 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     }
     fp.Phi = pl.Phi;
     fp.Plo = pl.Plo;
-    HIPCHK(launch_crt_final(fp, (const TileWork*)ctx->oz_work.p, (int)work.size(), ctx->stream));
+    st = segment([&]() { return launch_crt_final(fp, (const TileWork*)b_work.p, (int)work.size(), ctx->stream); });
+    if (st) return st;
     // the host vectors above must outlive the async copies
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *ms_igemm = 0.0;
@@ -357,7 +449,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
 // evaluated with the exact norms and the run repeated with more moduli if a
 // k-point misses the target.  The same samples measure the actual errors.
 hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<OzK>& ks, int flags, int icfg,
-                       int nmod_req, double target, std::vector<OzResult>& res)
+                       int nmod_req, double target, std::vector<OzResult>& res, bool capture)
 {
     const IgemmLaunchCfg ic = igemm_cfg(icfg);
     const int64_t kd = roundup(geo.kpad, ic.bk);           // INT8 contraction depth (complex)
@@ -539,7 +631,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
             for (int k = 0; k < nk; ++k)
                 if (todo[k] && nm[k] == n) sub.push_back(ks[k]);
             double ms_ig = 0.0;
-            st = oz_execute(ctx, geo, sub, flags, icfg, n, eZ, eZs, eW, &ms_ig);
+            st = oz_execute(ctx, geo, sub, flags, icfg, n, eZ, eZs, eW, &ms_ig, capture);
             if (st) return st;
             // each k-point's share of the shared launches, by its work (ng^2)
             double w = 0.0;
@@ -803,7 +895,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
             HIPCHK(launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)ncols, ctx->stream));
         }
         std::vector<OzResult> ozr;
-        st = run_ozaki(ctx, geo, ks, flags, icfg, o.slices, o.target_rel_err, ozr);
+        st = run_ozaki(ctx, geo, ks, flags, icfg, o.slices, o.target_rel_err, ozr, true);
         if (st) return st;
         for (size_t q = 0; q < bk.size() && (flags & HSGEN_FILL_UPPER); ++q) {
             if (do_h) HIPCHK(launch_fill_upper(ks[q].dH, ks[q].ldH, ks[q].ng, ctx->stream));

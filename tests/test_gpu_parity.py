@@ -333,6 +333,40 @@ def test_batched_split_precision(ctx, monkeypatch, budget):
     assert reps[0]["ms_total"] > 0
 
 
+def test_batched_split_graph_replays(ctx):
+    """The batched split-precision call captures its INT8 launch sequence
+    (residue passes, INT8 launches, recombination) into hipGraphs on the
+    first call and replays them on the next ones: with eager single-k calls
+    and synchronizations between the replays, every replay gives the eager
+    single-k results bit for bit; another G list (same sizes) captures anew."""
+    import torch
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
+    kps = _kset(s, nk=4, seed=11)
+    h = ctx.upload(s)
+    dev = [(torch.zeros((kp.ng, kp.ng), dtype=torch.complex128, device="cuda"),
+            torch.zeros((kp.ng, kp.ng), dtype=torch.complex128, device="cuda")) for kp in kps]
+    outs = [(a.data_ptr(), b.data_ptr(), kp.ng) for (a, b), kp in zip(dev, kps)]
+    eager = [ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12) for kp in kps]
+    for rep in range(3):                 # capture, then two replays
+        for a, b in dev:
+            a.zero_()
+            b.zero_()
+        _, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12,
+                                         outputs=outs)
+        torch.cuda.synchronize()
+        for (a, b), (H1, S1), kp in zip(dev, eager, kps):
+            assert np.array_equal(a.cpu().numpy().T, H1) and np.array_equal(b.cpu().numpy().T, S1), (rep, kp.ng)
+        assert all(r["ms_igemm"] > 0 for r in reps)
+        # eager work on the same context between the replays
+        ctx.create_HS2(h, kps[rep % len(kps)], precision=HSGEN_PREC_OZAKI_INT8, slices=9)
+    kps[2].g = kps[2].g[::-1].copy()
+    ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, outputs=outs)
+    H1, S1 = ctx.create_HS2(h, kps[2], precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    assert np.array_equal(np.tril(dev[2][0].cpu().numpy().T), H1)
+    assert np.array_equal(np.tril(dev[2][1].cpu().numpy().T), S1)
+
+
 def test_batched_split_fixed_moduli_and_flags(ctx):
     """A fixed modulus count for every k-point, H or S skipped or the upper
     triangles filled: the same bits as single-k calls (a skipped matrix stays
@@ -388,3 +422,13 @@ def test_tuning_knobs_keep_the_bits(ctx, monkeypatch):
     res, _ = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
     for (Ha, Sa), (Hb, Sb) in zip(ref, res):
         assert np.array_equal(Ha, Hb) and np.array_equal(Sa, Sb)
+    monkeypatch.delenv("HSGEN_RESIDUE_PASSES")
+    # the tile order (patch side, XCD remap) changes which tiles run together only
+    for env in (("HSGEN_PATCH", "8"), ("HSGEN_PATCH", "1"), ("HSGEN_XCD_REMAP", "0")):
+        monkeypatch.setenv(*env)
+        H1, S1 = ctx.create_HS2(h, kps[0])
+        assert np.array_equal(H0, H1) and np.array_equal(S0, S1), env
+        res, _ = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+        for (Ha, Sa), (Hb, Sb) in zip(ref, res):
+            assert np.array_equal(Ha, Hb) and np.array_equal(Sa, Sb), env
+        monkeypatch.delenv(env[0])
