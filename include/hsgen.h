@@ -210,8 +210,9 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G,
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * call's workspaces (a host G list's device copy, the grouped-GEMM work
- * lists, the operands Z, Zs, W: 3 x 16 B x (2K + 32) x roundup(ng, 128) for
- * the default tile) live in one device array of HSGEN_NOALLOC_SCRATCH_BYTES
+ * lists, the operands: 2 x 16 B x (2K + 32) x roundup(ng, 128) for the
+ * default tile, 3 x for a system with a zero ||u_dot||^2) live in one device
+ * array of HSGEN_NOALLOC_SCRATCH_BYTES
  * per GPU, held by the library until the process ends; a problem that needs
  * more is HSGEN_ERR_ARG.  H and S must be device memory (HSGEN_ERR_ARG
  * otherwise); kp->g may be host or device memory.  opts->precision must be
