@@ -262,7 +262,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     hsgen_ctx::OzGraph* cg = nullptr;
     bool fresh = false;
     if (capture) {
-        std::vector<int64_t> key{icfg, nmod, group, flags, (int64_t)geo.fold, geo.ldz, kd,
+        std::vector<int64_t> key{icfg, nmod, group, flags, (int64_t)geo.fold, geo.ldz, kd, gp,
                                  (int64_t)(intptr_t)Z, (int64_t)(intptr_t)Zs, (int64_t)(intptr_t)W,
                                  (int64_t)(intptr_t)ctx->oz_res.p, (int64_t)(intptr_t)rout, (int64_t)(intptr_t)eZ,
                                  (int64_t)(intptr_t)eZs, (int64_t)(intptr_t)eW};

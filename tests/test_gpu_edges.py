@@ -177,6 +177,12 @@ def test_unfolded_operands_when_a_norm_is_zero(ctx, precision):
     Hc, Sc = ctx.create_HS2_stripped_v2(s.L, s.natoms, kp.ng, *po.construct_ab(s, kp), s.taa, s.tab, s.tbb, s.lmax,
                                         s.max_lmax, un, precision=prec, target_rel_err=1e-12)
     assert rel_fro_lower(Hc, Hr) <= tol and rel_fro_lower(Sc, Sr) <= tol
+    # the batched entry point on the same three-array operands: the single-k bits
+    sub = _subset(kp, np.arange(0, kp.ng, 3))
+    res, _ = ctx.create_HS2_batched(h, [kp, sub], precision=prec, target_rel_err=1e-12)
+    for (Hb, Sb), k in zip(res, [kp, sub]):
+        H1, S1 = ctx.create_HS2(h, k, precision=prec, target_rel_err=1e-12)
+        assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
 
 
 def test_folded_and_unfolded_operands_agree(ctx, monkeypatch, big_list):
@@ -257,6 +263,15 @@ def test_argument_errors(ctx, big_list):
     # too few moduli for the contraction depth: an accuracy error, not a wrong result
     with pytest.raises(HsgenError) as e:
         ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, slices=2)
+    assert e.value.status == -6
+    # a target below what 52 bits per element reach: an accuracy error, in
+    # the single-k and in the batched entry point
+    with pytest.raises(HsgenError) as e:
+        ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-20)
+    assert e.value.status == -6 and "reaches rel err" in str(e.value)
+    with pytest.raises(HsgenError) as e:
+        ctx.create_HS2_batched(h, [sub, _subset(kp, np.arange(7))], precision=HSGEN_PREC_OZAKI_INT8,
+                               target_rel_err=1e-20)
     assert e.value.status == -6
     # null outputs
     lib = _lib.load()
