@@ -175,10 +175,10 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
                 rf[4 * l + 3] = ci * Bu;
             }
         }
-        if (kCompact && tid == 0) {
-            int acc = 0;
-            for (int t = 0; t < na; ++t) { s_off[t] = acc; acc += 2 * p.atoms[a0 + t].nlm; }
-            s_off[na] = acc;
+        if (kCompact && tid <= na) {
+            // rows of the chunk's atoms from their global offsets, one lane each
+            const int o = a0 + tid < p.natoms ? p.atoms[a0 + tid].off : p.kdim;
+            s_off[tid] = 2 * (o - p.atoms[a0].off);
         }
         __syncthreads();
         // ---- phase 1b: conj(Y_lm) by recurrence in l for each (atom, m >= 0) ----
@@ -363,7 +363,8 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
     double* s_dir = s_bes + p.nrmt * 2 * nl;             // [nrot][4]    cth sth cph sph
     double* s_rf = s_dir + p.nrot * 4;                   // [chunk][nl][4]
     double* s_un = s_rf + chunk * nl * 4;                // [chunk][nl]  B-row scale (Zs, or Z folded)
-    int* s_off = (int*)(s_un + chunk * nl);              // [chunk+1]    row prefix (compact)
+    double* s_ph = s_un + chunk * nl;                    // [chunk][2]   cos, sin of the phase 2 pi K.tau
+    int* s_off = (int*)(s_ph + chunk * 2);               // [chunk+1]    row prefix (compact)
     int* s_rc = s_off + chunk + 1;                       // [chunk]      rotation class
 
     if (kCompact && col >= p.ng) {
@@ -437,16 +438,26 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
 
     for (int a0 = 0; a0 < p.natoms; a0 += chunk) {
         const int na = min(chunk, p.natoms - a0);
+        // per atom, one lane each: the phase e^{i 2 pi K.tau}, the rotation
+        // class and the atom's first row in the chunk (compact layout)
+        if (tid < na) {
+            const AtomDev& at = p.atoms[a0 + tid];
+            const double kp = K0 * at.pos[0] + K1 * at.pos[1] + K2 * at.pos[2];
+            sincos(2.0 * kPi * kp, &s_ph[2 * tid + 1], &s_ph[2 * tid]);
+            s_rc[tid] = at.cls & 0xffff;
+        }
+        if (kCompact && tid <= na) {
+            const int o = a0 + tid < p.natoms ? p.atoms[a0 + tid].off : p.kdim;
+            s_off[tid] = 2 * (o - p.atoms[a0].off);
+        }
+        __syncthreads();
         // radial factors (common * a_l, common * b_l) per (atom, l)
         for (int q = tid; q < na * nl; q += blockDim.x) {
             const int t = q / nl;
             const int l = q - t * nl;
             const AtomDev& at = p.atoms[a0 + t];
-            if (l == 0) s_rc[t] = at.cls & 0xffff;
             if (l > at.lmax) continue;
-            const double kp = K0 * at.pos[0] + K1 * at.pos[1] + K2 * at.pos[2];
-            double ps, pc;
-            sincos(2.0 * kPi * kp, &ps, &pc);
+            const double pc = s_ph[2 * t], ps = s_ph[2 * t + 1];
             const double* fj = s_bes + (at.cls >> 16) * 2 * nl;
             const double* dfj = fj + nl;
             const RadDev m = p.rad[(int64_t)(a0 + t) * nl + l];
@@ -460,14 +471,6 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
             rf[2] = cr * Bu;
             rf[3] = ci * Bu;
             s_un[t * nl + l] = p.fold ? m.sqrt_un : m.dE_u_norm;
-        }
-        if (kCompact && tid == 0) {
-            int acc = 0;
-            for (int t = 0; t < na; ++t) {
-                s_off[t] = acc;
-                acc += 2 * p.atoms[a0 + t].nlm;
-            }
-            s_off[na] = acc;
         }
         __syncthreads();
         if (kCompact) {
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
 size_t abgen_cls_lds_bytes(int chunk, int nl, int L, int nrot, int nrmt)
 {
     return 16 * (size_t)nrot * L +
-           sizeof(double) * ((size_t)nrmt * 2 * nl + (size_t)nrot * 4 + (size_t)chunk * nl * 5) +
+           sizeof(double) * ((size_t)nrmt * 2 * nl + (size_t)nrot * 4 + (size_t)chunk * nl * 5 + (size_t)chunk * 2) +
            sizeof(int) * (size_t)(2 * chunk + 1);
 }
 
