@@ -820,6 +820,12 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
 // lower-triangle element inside the matrix, the residues of all moduli in
 // modulus order go into a double-double sum of r_m u_m / p_m reduced mod 1
 // (its centred fraction is C' / P), then C = 2^(eL_i + eR_j - 2 beta) P frac.
+//
+// A lane takes 4 consecutive slot positions, which the fragment order puts in
+// one row and 4 consecutive columns: per modulus one 8-byte load, all moduli
+// loaded before the sums start (8 independent double-double chains), and the
+// 16 rows x 16 columns that a wave covers are stored as 256 contiguous bytes
+// per column.
 __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWork* __restrict__ work)
 {
 #pragma clang fp contract(off)
@@ -827,11 +833,11 @@ __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWo
     const OzFinalProblem pb = p.prob[tw.prob];
     const int wm_n = p.bm / p.wgm, wn_n = p.bn / p.wgn;     // wave tile
     const int RB = wm_n / 16, CB = wn_n / 16;
-    const int tile = p.bm * p.bn;
-    const uchar2* r0 = (const uchar2*)p.res + (int64_t)tw.slot * tile;
-    const int64_t mstride = p.mod_stride / 2;
-    for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-        const int lane = e & 63;
+    const int tile = p.bm * p.bn;                           // a multiple of 256
+    const uint8_t* r0 = p.res + (int64_t)tw.slot * tile * 2;
+    const int nmod = p.nmod;
+    for (int e = 4 * threadIdx.x; e < tile; e += 4 * blockDim.x) {
+        const int lane = e & 63;                            // a multiple of 4
         int t = e >> 6;
         const int q = t & 3;
         t >>= 2;
@@ -840,39 +846,60 @@ __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWo
         const int i = t % RB;
         const int wave = t / RB;
         const int gi = tw.bi * p.bm + (wave % p.wgm) * wm_n + i * 16 + 4 * (lane >> 4) + q;
-        const int gj = tw.bj * p.bn + (wave / p.wgm) * wn_n + j * 16 + (lane & 15);
-        if (gi >= pb.m || gj >= pb.n || gi < gj) continue;
-        double hh[2] = {0.0, 0.0}, ll[2] = {0.0, 0.0};
-        for (int m = 0; m < p.nmod; ++m) {
-            const uchar2 r = r0[m * mstride + e];
-            dd_accum(hh[0], ll[0], (double)r.x, p.whi[m], p.wlo[m], m == 0);
-            dd_accum(hh[1], ll[1], (double)r.y, p.whi[m], p.wlo[m], m == 0);
-        }
-        const int sh = pb.eL[gi] + pb.eR[gj] - 2 * p.beta;
-        double out[2];
+        const int gj0 = tw.bj * p.bn + (wave / p.wgm) * wn_n + j * 16 + (lane & 15);
+        // columns gj0 .. gj0 + 3 of row gi; none is kept when the first is not
+        if (gi >= pb.m || gj0 >= pb.n || gi < gj0) continue;
+        uint2 rv[kMaxModuli];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            // the sum is already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
-            double fh = hh[c], fl = ll[c];
-            const double n = rint(fh);
-            fh -= n;
-            double s, err0;
-            two_sum(fh, fl, s, err0);
-            // (s + err0) * (Phi + Plo)
-            const double v = s * p.Phi;
-            double err = fma(s, p.Phi, -v);
-            err = fma(s, p.Plo, err);
-            err = fma(err0, p.Phi, err);
-            out[c] = ldexp(v + err, sh);
+        for (int m = 0; m < kMaxModuli; ++m)
+            rv[m] = *(const uint2*)(r0 + (int64_t)min(m, nmod - 1) * p.mod_stride + 2 * e);
+        // c = 2 k + part: element gj0 + k, real (0) or imaginary (1) part
+        double hh[8], ll[8];
+#pragma unroll
+        for (int m = 0; m < kMaxModuli; ++m) {
+            if (m < nmod) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const uint32_t w = c < 4 ? rv[m].x : rv[m].y;
+                    const double r = (double)((w >> (8 * (c & 3))) & 0xffu);
+                    dd_accum(hh[c], ll[c], r, p.whi[m], p.wlo[m], m == 0);
+                }
+            }
         }
-        if (gi == gj) out[1] = 0.0;
-        pb.C[(int64_t)gj * pb.ldc + gi] = make_double2(out[0], out[1]);
+        const int eLi = pb.eL[gi];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int gj = gj0 + k;
+            if (gj >= pb.n || gi < gj) continue;
+            const int sh = eLi + pb.eR[gj] - 2 * p.beta;
+            double out[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                // the sum is already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
+                double fh = hh[2 * k + c], fl = ll[2 * k + c];
+                const double n = rint(fh);
+                fh -= n;
+                double s, err0;
+                two_sum(fh, fl, s, err0);
+                // (s + err0) * (Phi + Plo)
+                const double v = s * p.Phi;
+                double err = fma(s, p.Phi, -v);
+                err = fma(s, p.Plo, err);
+                err = fma(err0, p.Phi, err);
+                out[c] = ldexp(v + err, sh);
+            }
+            if (gi == gj) out[1] = 0.0;
+            pb.C[(int64_t)gj * pb.ldc + gi] = make_double2(out[0], out[1]);
+        }
     }
 }
 
 hipError_t launch_crt_final(const OzFinalParams& p, const TileWork* work, int nwork, hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
+    // 8-byte residue loads of 4 slot positions; every modulus slot read
+    if (p.nmod < 1 || p.nmod > kMaxModuli || (p.bm * p.bn) % 256 != 0 || p.mod_stride % 8 != 0)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_crt_final, dim3(nwork), dim3(256), 0, st, p, work);
     return hipGetLastError();
 }
