@@ -177,8 +177,9 @@ struct OzFinalParams {
     const OzFinalProblem* prob; // [problem id of the INT8 work list], device memory
     int32_t nmod, beta;
     int32_t bm, bn, wgm, wgn;   // INT8 tile: the layout of a slot
-    double whi[kMaxModuli];     // u_m / p_m as double-double (u_m = (P / p_m)^-1 mod p_m)
-    double wlo[kMaxModuli];
+    // u_m / p_m = w1 + w2 + w3 (u_m = (P / p_m)^-1 mod p_m): w1 a multiple of
+    // 2^-40, w2 of 2^-80 below 2^-40, w3 below 2^-80
+    double w1[kMaxModuli], w2[kMaxModuli], w3[kMaxModuli];
     double Phi, Plo;            // product of the moduli as double-double
 };
 

@@ -232,42 +232,15 @@ hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st)
 // ---------------------------------------------------------------------------
 // 3. INT8 MFMA GEMM with CRT epilogue
 // ---------------------------------------------------------------------------
-// double-double helpers.  Error-free transformations: no multiply-add
-// contraction may touch them (hipcc contracts by default, and an a*b + c
-// fused into one rounding breaks the error term they compute).
+// Error-free sum of two doubles.  No multiply-add contraction may touch it
+// (hipcc contracts by default, and a*b + c fused into one rounding breaks the
+// error term it computes).
 __device__ inline void two_sum(double a, double b, double& s, double& e)
 {
 #pragma clang fp contract(off)
     s = a + b;
     double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
-}
-__device__ inline void fast_two_sum(double a, double b, double& s, double& e)
-{
-#pragma clang fp contract(off)
-    s = a + b;
-    e = b - (s - a);
-}
-
-// (hi, lo) <- frac((hi, lo) + r * (whi, wlo)), |frac| <= 1/2
-__device__ inline void dd_accum(double& hi, double& lo, double r, double whi, double wlo, bool first)
-{
-#pragma clang fp contract(off)
-    double ph = r * whi;
-    double pl = fma(r, whi, -ph);
-    pl = fma(r, wlo, pl);
-    double s, e;
-    if (first) {
-        s = ph;
-        e = pl;
-    } else {
-        two_sum(hi, ph, s, e);
-        e += lo + pl;
-    }
-    fast_two_sum(s, e, s, e);
-    const double n = rint(s);
-    s -= n;                              // exact
-    two_sum(s, e, hi, lo);               // s may now be smaller than e
 }
 
 // Position of a tile's element in its block of residue outputs: the order in
@@ -817,15 +790,21 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
 // 5. CRT finalisation: C = 2^(eL_i + eR_j - 2 beta) * P * centred_frac(acc)
 // ---------------------------------------------------------------------------
 // Recombination, one workgroup per tile of the INT8 work list: for each
-// lower-triangle element inside the matrix, the residues of all moduli in
-// modulus order go into a double-double sum of r_m u_m / p_m reduced mod 1
-// (its centred fraction is C' / P), then C = 2^(eL_i + eR_j - 2 beta) P frac.
+// lower-triangle element inside the matrix, the residues r_m in [0, p_m) of
+// all moduli give C' / P = centred_frac(sum_m r_m w_m), w_m = u_m / p_m,
+// u_m = (P / p_m)^-1 mod p_m; then C = 2^(eL_i + eR_j - 2 beta) P frac.
+//
+// The host splits each weight as w_m = w1_m + w2_m + w3_m with w1_m a multiple
+// of 2^-40 below 1, w2_m a multiple of 2^-80 below 2^-40 and w3_m < 2^-80.
+// Then with r_m <= 254 and at most 20 moduli, sum r_m w1_m is a multiple of
+// 2^-40 below 2^13 and sum r_m w2_m a multiple of 2^-80 below 2^-27.6: both
+// fit 53 bits, so their fma chains are exact; only the third sum rounds, at
+// ~2^-120 absolute.  Three fmas per residue.
 //
 // A lane takes 4 consecutive slot positions, which the fragment order puts in
 // one row and 4 consecutive columns: per modulus one 8-byte load, all moduli
-// loaded before the sums start (8 independent double-double chains), and the
-// 16 rows x 16 columns that a wave covers are stored as 256 contiguous bytes
-// per column.
+// loaded before the sums start, and the 16 rows x 16 columns that a wave
+// covers are stored as 256 contiguous bytes per column.
 __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWork* __restrict__ work)
 {
 #pragma clang fp contract(off)
@@ -854,7 +833,13 @@ __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWo
         for (int m = 0; m < kMaxModuli; ++m)
             rv[m] = *(const uint2*)(r0 + (int64_t)min(m, nmod - 1) * p.mod_stride + 2 * e);
         // c = 2 k + part: element gj0 + k, real (0) or imaginary (1) part
-        double hh[8], ll[8];
+        double s1[8], s2[8], s3[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            s1[c] = 0.0;
+            s2[c] = 0.0;
+            s3[c] = 0.0;
+        }
 #pragma unroll
         for (int m = 0; m < kMaxModuli; ++m) {
             if (m < nmod) {
@@ -862,7 +847,9 @@ __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWo
                 for (int c = 0; c < 8; ++c) {
                     const uint32_t w = c < 4 ? rv[m].x : rv[m].y;
                     const double r = (double)((w >> (8 * (c & 3))) & 0xffu);
-                    dd_accum(hh[c], ll[c], r, p.whi[m], p.wlo[m], m == 0);
+                    s1[c] = fma(r, p.w1[m], s1[c]);         // exact
+                    s2[c] = fma(r, p.w2[m], s2[c]);         // exact
+                    s3[c] = fma(r, p.w3[m], s3[c]);
                 }
             }
         }
@@ -875,17 +862,19 @@ __global__ __launch_bounds__(256) void k_crt_final(OzFinalParams p, const TileWo
             double out[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                // the sum is already centred in [-1/2, 1/2] (dd_accum reduces mod 1)
-                double fh = hh[2 * k + c], fl = ll[2 * k + c];
-                const double n = rint(fh);
-                fh -= n;
+                const int cc = 2 * k + c;
+                const double f1 = s1[cc] - rint(s1[cc]);   // exact, in [-1/2, 1/2]
                 double s, err0;
-                two_sum(fh, fl, s, err0);
-                // (s + err0) * (Phi + Plo)
-                const double v = s * p.Phi;
-                double err = fma(s, p.Phi, -v);
-                err = fma(s, p.Plo, err);
-                err = fma(err0, p.Phi, err);
+                two_sum(f1, s2[cc], s, err0);
+                err0 += s3[cc];
+                s -= rint(s);                              // exact: back into [-1/2, 1/2]
+                double h, l;
+                two_sum(s, err0, h, l);
+                // (h + l) * (Phi + Plo)
+                const double v = h * p.Phi;
+                double err = fma(h, p.Phi, -v);
+                err = fma(h, p.Plo, err);
+                err = fma(l, p.Phi, err);
                 out[c] = ldexp(v + err, sh);
             }
             if (gi == gj) out[1] = 0.0;

@@ -33,7 +33,7 @@ struct OzPlan {
     int beta = 0;
     double log2P = 0.0;
     double Phi = 0.0, Plo = 0.0;
-    double whi[kMaxModuli], wlo[kMaxModuli];
+    double w1[kMaxModuli], w2[kMaxModuli], w3[kMaxModuli];   // u_m / p_m in three parts
 };
 
 int inv_mod(int a, int m)
@@ -86,11 +86,15 @@ OzPlan oz_plan(int nmod)
         int u = 1;
         for (int k = 0; k < nmod; ++k)
             if (k != m) u = (u * inv_mod(kModuli[k], kModuli[m])) % kModuli[m];
-        const double p = kModuli[m];
-        const double hi = (double)u / p;
-        const double r = std::fma(-hi, p, (double)u);   // exact remainder
-        pl.whi[m] = hi;
-        pl.wlo[m] = r / p;
+        // u / p = k1 2^-40 + k2 2^-80 + (r2 / p) 2^-80, by integer long division
+        const int64_t p = kModuli[m];
+        const int64_t a1 = (int64_t)u << 40;                // < 2^48
+        const int64_t k1 = a1 / p, r1 = a1 - k1 * p;
+        const int64_t a2 = r1 << 40;
+        const int64_t k2 = a2 / p, r2 = a2 - k2 * p;
+        pl.w1[m] = std::ldexp((double)k1, -40);             // k1, k2 < 2^40: exact
+        pl.w2[m] = std::ldexp((double)k2, -80);
+        pl.w3[m] = std::ldexp((double)r2 / (double)p, -80);
     }
     return pl;
 }
@@ -420,8 +424,9 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     fp.wgm = ic.wgm;
     fp.wgn = ic.wgn;
     for (int m = 0; m < nmod; ++m) {
-        fp.whi[m] = pl.whi[m];
-        fp.wlo[m] = pl.wlo[m];
+        fp.w1[m] = pl.w1[m];
+        fp.w2[m] = pl.w2[m];
+        fp.w3[m] = pl.w3[m];
     }
     fp.Phi = pl.Phi;
     fp.Plo = pl.Plo;
