@@ -125,6 +125,7 @@ struct OzResParams {
     int32_t nmod;
     int32_t p[kMaxModuli];
     double invp[kMaxModuli];
+    double c24[kMaxModuli];      // 2^24 mod p, symmetric
     int8_t* out[kMaxModuli];     // (re, im) residue pairs, column stride ldo bytes, or null
     int8_t* out_rot[kMaxModuli]; // rotated pairs or null
     int32_t rot_kind;            // 1: (-im, re) (left operand), 2: (im, -re) (right operand)

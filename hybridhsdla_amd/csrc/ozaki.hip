@@ -174,14 +174,32 @@ hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int
 // ---------------------------------------------------------------------------
 // 2. residues
 // ---------------------------------------------------------------------------
-// symmetric residue of an integer-valued double |x| <= 2^53 modulo p
-__device__ inline int sym_mod(double x, double p, double invp, int lo, int hi)
+// Symmetric residues.  An integer-valued double |x| < 2^53 is split once into
+// x = xh 2^24 + xl (xl in [0, 2^24)); per modulus y = xh c + xl with
+// c = 2^24 mod p in [-127, 127] is congruent to x and |y| < 2^36, so y / p is
+// known to ~2^-23 and one rounding finds the symmetric residue y - p rint(y/p).
+// Adding 2^-20 before the rounding sends the exact ties of the even modulus
+// (y / p = k + 1/2) up, to -p/2 = lo; every other y / p lies at least 1/(2p)
+// from a rounding boundary.  Five operations per value and modulus.
+__device__ inline void split24(double x, double& xh, double& xl)
 {
-    double q = rint(x * invp);
-    int r = (int)fma(-q, p, x);     // exact: |q p| < 2^53
-    if (r > hi) r -= (int)p;
-    if (r < lo) r += (int)p;
-    return r;
+    xh = floor(x * 0x1p-24);
+    xl = fma(-xh, 0x1p24, x);                 // exact
+}
+
+__device__ inline int sym_res(double xh, double xl, double c, double p, double invp)
+{
+    const double y = fma(xh, c, xl);          // exact: |y| < 2^36
+    const double q = rint(fma(y, invp, 0x1p-20));
+    return (int)fma(-q, p, y);                // exact
+}
+
+// bytes 0 of a, b, c, d -> one dword (a in byte 0)
+__device__ inline uint32_t pack_bytes(int a, int b, int c, int d)
+{
+    const uint32_t ab = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+    const uint32_t cd = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x04000c0cu);
+    return ab | cd;
 }
 
 // One lane per 4 consecutive rows of a column: 4 complex values in, per
@@ -193,31 +211,35 @@ __global__ __launch_bounds__(256) void k_residues(OzResParams p)
     if (r0 >= p.rows) return;
     const int sh = p.beta - p.e[col];
     const double2* xc = p.X + (int64_t)col * p.ldx;
-    double xr[4], xi[4];
+    double hr[4], lr[4], hi_[4], li[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double2 v = r0 + k < p.rows_src ? xc[r0 + k] : make_double2(0.0, 0.0);
-        xr[k] = rint(ldexp(v.x, sh));
-        xi[k] = rint(ldexp(v.y, sh));
+        split24(rint(ldexp(v.x, sh)), hr[k], lr[k]);
+        split24(rint(ldexp(v.y, sh)), hi_[k], li[k]);
     }
     for (int m = 0; m < p.nmod; ++m) {
-        const double pm = (double)p.p[m];
-        const int hi = (p.p[m] - 1) / 2;
-        const int lo = -(p.p[m] / 2);
-        uint32_t w[2] = {0u, 0u}, wr[2] = {0u, 0u};
+        const double pm = (double)p.p[m], ip = p.invp[m], c = p.c24[m];
+        int rr[4], ri[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int rr = sym_mod(xr[k], pm, p.invp[m], lo, hi);
-            const int ri = sym_mod(xi[k], pm, p.invp[m], lo, hi);
-            // bytes (re, im) of row r0 + k at byte offset 2 k
-            const int s = 16 * (k & 1);
-            w[k >> 1] |= ((uint32_t)(rr & 0xff) | ((uint32_t)(ri & 0xff) << 8)) << s;
-            const int a = p.rot_kind == 1 ? -ri : ri;
-            const int b = p.rot_kind == 1 ? rr : -rr;
-            wr[k >> 1] |= ((uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8)) << s;
+            rr[k] = sym_res(hr[k], lr[k], c, pm, ip);
+            ri[k] = sym_res(hi_[k], li[k], c, pm, ip);
         }
-        if (p.out[m]) *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(w[0], w[1]);
-        if (p.out_rot[m]) *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) = make_uint2(wr[0], wr[1]);
+        // bytes (re, im) of row r0 + k at byte offset 2 k
+        if (p.out[m])
+            *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) =
+                make_uint2(pack_bytes(rr[0], ri[0], rr[1], ri[1]), pack_bytes(rr[2], ri[2], rr[3], ri[3]));
+        if (p.out_rot[m]) {
+            int a[4], b[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                a[k] = p.rot_kind == 1 ? -ri[k] : ri[k];
+                b[k] = p.rot_kind == 1 ? rr[k] : -rr[k];
+            }
+            *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) =
+                make_uint2(pack_bytes(a[0], b[0], a[1], b[1]), pack_bytes(a[2], b[2], a[3], b[3]));
+        }
     }
 }
 

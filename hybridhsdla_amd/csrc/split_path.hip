@@ -361,6 +361,8 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
         for (int i = 0; i < gn; ++i) {
             rp.p[i] = kModuli[m0 + i];
             rp.invp[i] = 1.0 / kModuli[m0 + i];
+            const int c = (1 << 24) % kModuli[m0 + i];
+            rp.c24[i] = c > (kModuli[m0 + i] - 1) / 2 ? c - kModuli[m0 + i] : c;
         }
         // residues of the k-points' columns of X into arrays a (and rotated arot)
         auto residues = [&](const double2* X, const int32_t* e, int a, int arot) -> hipError_t {
