@@ -189,6 +189,7 @@ hipError_t launch_colexp(const double2* X, int64_t ldx, int rows, int ncols, int
 hipError_t launch_tril_sumsq(const double2* C, int64_t ldc, int n, double* part, int nblk, hipStream_t st);
 hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
                             const int32_t* cols, int lower, double* part, hipStream_t st);
+hipError_t launch_sum_parts(double2* acc, const double2* parts, int64_t n, int nparts, hipStream_t st);
 hipError_t launch_gather_cols(const double2* X, int64_t ld, const int32_t* cols, int rows, int n, double2* Y,
                               hipStream_t st);
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st);

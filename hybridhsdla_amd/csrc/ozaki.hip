@@ -163,6 +163,30 @@ hipError_t launch_gather_cols(const double2* X, int64_t ld, const int32_t* cols,
     return hipGetLastError();
 }
 
+// acc[i] += parts[0][i], then parts[1][i], ... (in that order), n elements,
+// the parts `n` apart
+__global__ __launch_bounds__(256) void k_sum_parts(double2* __restrict__ acc, const double2* __restrict__ parts,
+                                                   int64_t n, int nparts)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = acc[i];
+        for (int c = 0; c < nparts; ++c) {
+            const double2 w = parts[c * n + i];
+            v.x += w.x;
+            v.y += w.y;
+        }
+        acc[i] = v;
+    }
+}
+
+hipError_t launch_sum_parts(double2* acc, const double2* parts, int64_t n, int nparts, hipStream_t st)
+{
+    if (n <= 0 || nparts <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)blocks), dim3(256), 0, st, acc, parts, n, nparts);
+    return hipGetLastError();
+}
+
 hipError_t launch_block_cmp(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
                             const int32_t* cols, int lower, double* part, hipStream_t st)
 {

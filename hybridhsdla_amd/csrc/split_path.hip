@@ -525,29 +525,42 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
     // FP64 samples: columns cols[] of S and H of each k-point, all rows, ld = ng
     // (the right operand's sampled columns are gathered first: the GEMM reads
     // contiguous columns, whole tiles of bn of them)
-    st = ensure(ctx, ctx->oz_smp, sizeof(double2) * std::max<size_t>(2 * (smpt + gatht), 1), false);
+    // The sample GEMM of one k-point has only ng / 64 tiles per product: at
+    // large depth it is split into nsplit depth chunks (one per 16384 rows, at
+    // most 8), so that the launch fills the chip; the chunks' products are
+    // summed in chunk order afterwards.  nsplit depends on the depth only, so
+    // a k-point's sample is the same in single and batched calls.
+    const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(8, geo.kpad / 16384));
+    const int64_t kchunk = roundup((geo.kpad + nsplit - 1) / nsplit, sc.bk);
+    st = ensure(ctx, ctx->oz_smp, sizeof(double2) * std::max<size_t>(2 * (smpt + gatht) + 2 * smpt * (nsplit - 1), 1),
+                false);
     if (st) return st;
     double2* smpS = (double2*)ctx->oz_smp.p;
     double2* smpH = smpS + smpt;
     double2* gathS = smpH + smpt;
     double2* gathH = gathS + gatht;
+    double2* smp_part = gathH + gatht;             // chunks 1 .. nsplit-1 of smpS and smpH, 2 smpt each
     {
         std::vector<GemmProblem> probs;
         std::vector<TileWork> work;
         auto add = [&](const OzK& k, const double2* R, double2* C, int n) {
-            GemmProblem p;
-            p.L = Z + k.col0 * geo.ldz;
-            p.R = R;
-            p.C = C;
-            p.ldl = p.ldr = geo.ldz;
-            p.ldc = k.ng;
-            p.m = k.ng;
-            p.n = n;
-            p.kdepth = (int32_t)geo.kpad;
-            p.lower = 0;
-            const int ntr = (k.ng + sc.bm - 1) / sc.bm;
-            for (int bi = 0; bi < ntr; ++bi) work.push_back(TileWork{(int)probs.size(), bi, 0, 0});
-            probs.push_back(p);
+            for (int c = 0; c < nsplit; ++c) {
+                const int64_t k0 = c * kchunk;
+                if (k0 >= geo.kpad) break;
+                GemmProblem p;
+                p.L = Z + k.col0 * geo.ldz + k0;
+                p.R = R + k0;
+                p.C = c == 0 ? C : smp_part + (size_t)(c - 1) * 2 * smpt + (C - smpS);
+                p.ldl = p.ldr = geo.ldz;
+                p.ldc = k.ng;
+                p.m = k.ng;
+                p.n = n;
+                p.kdepth = (int32_t)std::min<int64_t>(kchunk, geo.kpad - k0);
+                p.lower = 0;
+                const int ntr = (k.ng + sc.bm - 1) / sc.bm;
+                for (int bi = 0; bi < ntr; ++bi) work.push_back(TileWork{(int)probs.size(), bi, 0, 0});
+                probs.push_back(p);
+            }
         };
         for (int k = 0; k < nk; ++k) {
             if (nb[k] == 0) continue;
@@ -565,6 +578,7 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
         }
         st = run_grouped(ctx, kSampleTile, probs, work);
         if (st) return st;
+        if (nsplit > 1) HIPCHK(launch_sum_parts(smpS, smp_part, 2 * (int64_t)smpt, nsplit - 1, ctx->stream));
     }
     for (int k = 0; k < nk; ++k) {
         if (nb[k] == 0) continue;
