@@ -11,6 +11,7 @@
 - repeated G-vectors (equal rows and columns of H and S);
 - batched k-points of ragged sizes, including one with a single G-vector and
   one with none, in FP64 and split precision;
+- a contraction depth at which the split-precision sample splits its depth;
 - argument errors come back as status codes with a message, never a crash.
 
 Tolerances as in test_gpu_parity.py: relative Frobenius error of the lower
@@ -238,6 +239,33 @@ def test_batched_split_ragged_sizes_equal_single_calls(ctx, big_list):
         print(f"ng={k.ng}: {r['slices_used']} moduli", flush=True)
         H1, S1 = ctx.create_HS2(h, k, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
         assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
+
+
+def test_split_precision_deep_contraction(ctx):
+    """Depth 2K >= 32768 (100 atoms with lmax 12): the split-precision FP64
+    sample runs as depth chunks (two here) summed in chunk order.  The target
+    holds against the oracle, the sampled error matches the measured one on
+    the sampled columns' scale, and a batched call of two k-points gives the
+    single calls' bits and report fields (the chunking depends on the depth
+    alone)."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=100, lmax=(12,), a_cell=14.0, ng_target=200, seed=4)
+    assert 2 * s.kdim >= 32768
+    kps = [kp, _subset(kp, np.arange(kp.ng // 2) * 2)]
+    h = ctx.upload(s)
+    res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    for k, (Hb, Sb), rb in zip(kps, res, reps):
+        H, S, r = ctx.create_HS2(h, k, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+        Hr, Sr = po.reference_hs(s, k)
+        eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
+        print(f"ng={k.ng} K={s.kdim}: {r['slices_used']} moduli, rel err H {eh:.3e} S {es:.3e}, "
+              f"sampled {r['err_smp_h']:.3e} / {r['err_smp_s']:.3e}", flush=True)
+        assert eh <= 1e-12 and es <= 1e-12
+        # 64 sampled columns of ~200 rows: the sampled error within a factor 2 of the whole one
+        assert 0.5 * eh <= r["err_smp_h"] <= 2 * eh and 0.5 * es <= r["err_smp_s"] <= 2 * es
+        assert np.array_equal(Hb, H) and np.array_equal(Sb, S), k.ng
+        for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
+            assert rb[key] == r[key], (k.ng, key)
 
 
 def test_argument_errors(ctx, big_list):
