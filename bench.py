@@ -216,7 +216,7 @@ def lower_rel_err(X, ref, rows=2048):
     return (num / den) ** 0.5 if den > 0 else 0.0
 
 
-def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, warmup, idx):
+def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, warmup, idx, ceil=None):
     """Split precision (INT8 Ozaki) on the same k-point at the error target,
     timed like the FP64 run, with its error against the FP64 result on the
     whole lower triangle; returns the entry and the result's principal
@@ -265,6 +265,7 @@ def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, 
             "avg_launch_ms": round(ms_ig / max(nmod, 1), 3),
         },
     }
+    with_ceiling(entry["roofline"], ceil)
     return entry, smp
 
 
@@ -327,6 +328,8 @@ def bench_batched(args) -> int:
                 "achieved": round(ach, 1), "peak": INT8_PEAK, "unit": "TOP/s", "frac": round(ach / INT8_PEAK, 4),
                 "traffic": None, "ms_igemm_per_step": round(ms_ig, 2),
                 "moduli": sorted({int(x["slices_used"]) for x in reps[-1]})}
+        ceil = int8_ceiling(ctx)
+        with_ceiling(roof, ceil)
     check = None
     if not args.no_check:
         try:
@@ -374,6 +377,28 @@ def bench_batched(args) -> int:
     sysh.close()
     ctx.close()
     return 0
+
+
+def int8_ceiling(ctx):
+    """The INT8 MFMA ceiling of this board: the split-precision GEMMs' MFMA
+    instruction alone on every CU, random operands (hsgen_int8_mfma_ceiling),
+    which the board's power limit holds below the nominal peak."""
+    try:
+        ctx.int8_mfma_ceiling(30000)               # warm-up
+        tops, ms = ctx.int8_mfma_ceiling(0)
+        return {"achieved": round(tops, 1), "unit": "TOP/s", "frac_of_peak": round(tops / INT8_PEAK, 4),
+                "ms": round(ms, 1),
+                "kernel": "k_int8_mfma_ceiling (v_mfma_i32_16x16x64_i8 only, random operands, every CU)"}
+    except Exception as e:
+        return {"error": str(e)}
+
+
+def with_ceiling(roof, ceil):
+    """Add the board's INT8 ceiling and the fraction of it reached to an INT8 roofline."""
+    if roof and ceil and roof.get("achieved") and ceil.get("achieved"):
+        roof["ceiling"] = ceil["achieved"]
+        roof["frac_of_ceiling"] = round(roof["achieved"] / ceil["achieved"], 4)
+    return roof
 
 
 def load_traffic(kernel: str, workload: str):
@@ -507,6 +532,9 @@ def main(argv=None) -> int:
             "traffic": (load_traffic("igemm2_crt", workload_tag) or {}).get("bytes_per_launch"),
             "algorithmic_ops_per_launch": f_contract, "avg_launch_ms": round(ms_ig / nmod, 3),
         }
+    ceil = int8_ceiling(ctx) if (prec != HSGEN_PREC_FP64 or not args.no_split) else None
+    if prec != HSGEN_PREC_FP64:
+        with_ceiling(roofline, ceil)
     abgen_bytes = float(reps[-1]["bytes_abgen"])
     abgen = {
         "bound": "hbm", "kernel": "k_abgen_cls (A/B generator, Y_lm and Bessel values shared by rotation / rmt class)",
@@ -563,7 +591,8 @@ def main(argv=None) -> int:
                 # the headline target is timed like the FP64 run, the others
                 # with fewer steps
                 st, wu = (args.steps, args.warmup) if main_t else (min(args.steps, 3), min(args.warmup, 1))
-                entry, smp = split_run(args, system, G, H, S, H64, S64, step, ms_step, F, target, st, wu, idx)
+                entry, smp = split_run(args, system, G, H, S, H64, S64, step, ms_step, F, target, st, wu, idx,
+                                       ceil)
                 samples[f"split_{target:.0e}"] = (idx, *smp)
                 sweep.append(entry)
                 if main_t:
@@ -613,6 +642,7 @@ def main(argv=None) -> int:
         "self_check": check,
         "split_precision": split,
         "split_sweep": sweep,
+        "int8_mfma_ceiling": ceil,
         "host_outputs": host_out,
         "eigensolve": eig,
         "cpu_baseline": cpu,

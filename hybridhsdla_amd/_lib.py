@@ -103,6 +103,7 @@ SIGNATURES = [
      [_p, _p, C.POINTER(HsgenKpoint), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     ("hsgen_eigensolve", C.c_int,
      [_p, C.c_int32, _p, _p, C.c_int64, _p, _p, C.c_int64, C.POINTER(C.c_double)]),
+    ("hsgen_int8_mfma_ceiling", C.c_int, [_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("hsgen_radial_solve", C.c_int,
      [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, _p, _p, _p, _p]),
     ("hsgen_quadrature_weights", C.c_int, [C.c_int32, _p]),

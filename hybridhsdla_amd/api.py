@@ -220,6 +220,15 @@ class Context:
                                              n, C.byref(ms)), self.handle)
         return w, V, ms.value
 
+    def int8_mfma_ceiling(self, iters: int = 0):
+        """Diagnostic: sustained INT8 TOP/s of the split-precision GEMMs' MFMA
+        instruction with random operands and nothing else running (the
+        board's power-limited ceiling for them); returns (tops, gpu_ms)."""
+        tops, ms = C.c_double(), C.c_double()
+        _lib.check(self.lib.hsgen_int8_mfma_ceiling(self.handle, int(iters), C.byref(tops), C.byref(ms)),
+                   self.handle)
+        return tops.value, ms.value
+
     def create_HS2_batched(self, sysh: SystemHandle, kps: Sequence[KPoint], precision=HSGEN_PREC_FP64,
                            slices=0, target_rel_err=0.0, flags=0, tile=0, outputs=None):
         """H, S for many k-points; outputs = list of (H_ptr, S_ptr, ld) device

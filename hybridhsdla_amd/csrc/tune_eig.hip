@@ -1,5 +1,6 @@
-// The tile autotuner (hsgen_autotune) and the step after H/S generation, the
-// generalized eigensolve (hsgen_eigensolve).
+// The tile autotuner (hsgen_autotune), the step after H/S generation, the
+// generalized eigensolve (hsgen_eigensolve), and a diagnostic of the INT8
+// matrix pipe (hsgen_int8_mfma_ceiling).
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -19,6 +20,50 @@
 #include <rocsolver/rocsolver.h>
 
 namespace hsgen {
+
+namespace {
+
+__device__ inline int mix32(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return (int)x;
+}
+
+typedef int i4v_t __attribute__((ext_vector_type(4)));
+
+// Registers only: 12 independent accumulators per wave, operands that change
+// from one MFMA to the next (random bytes, as residues are).  The sustained
+// rate of v_mfma_i32_16x16x64_i8 with nothing else running sets the ceiling
+// the split-precision GEMMs meet at the board's power limit.
+constexpr int kCeilAcc = 12;
+__global__ __launch_bounds__(512) void k_int8_mfma_ceiling(int iters, int* sink)
+{
+    const unsigned t = threadIdx.x + blockIdx.x * 512u;
+    i4v_t a[6], b[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        a[k] = i4v_t{mix32(t * 97u + k), mix32(t * 89u + k + 7), mix32(t * 83u + k + 13), mix32(t * 79u + k + 29)};
+        b[k] = i4v_t{mix32(t * 71u + k + 3), mix32(t * 67u + k + 5), mix32(t * 61u + k + 11), mix32(t * 59u + k + 17)};
+    }
+    i4v_t acc[kCeilAcc];
+#pragma unroll
+    for (int k = 0; k < kCeilAcc; ++k) acc[k] = i4v_t{k, 0, 0, 0};
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < kCeilAcc; ++k)
+            acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k % 6], b[(k + k / 6) % 6], acc[k], 0, 0, 0);
+    }
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kCeilAcc; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    if (s == 0x7fffffff) sink[0] = s;   // keeps the products live; almost never stores
+}
+
+}  // namespace
 
 std::vector<int> tune_key(int precision, int ng, int kdim)
 {
@@ -150,6 +195,28 @@ hsgen_status hsgen_eigensolve(hsgen_ctx* ctx, int32_t n, const double* H, const 
         HIPCHK(hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]));
         *ms = tm;
     }
+    return HSGEN_OK;
+}
+
+// Diagnostic: the INT8 MFMA ceiling of this board (see k_int8_mfma_ceiling).
+hsgen_status hsgen_int8_mfma_ceiling(hsgen_ctx* ctx, int32_t iters, double* tops, double* ms)
+{
+    if (!ctx) return HSGEN_ERR_ARG;
+    if (iters <= 0) iters = 300000;
+    HIPCHK(hipSetDevice(ctx->device));
+    hsgen_status st = ensure(ctx, ctx->eig_d, std::max<size_t>(ctx->eig_d.bytes, sizeof(int)), false);
+    if (st) return st;
+    const int nblk = 8 * std::max(ctx->n_cu, 1);   // 8 workgroups of 8 waves per CU
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    hipLaunchKernelGGL(k_int8_mfma_ceiling, dim3(nblk), dim3(512), 0, ctx->stream, (int)iters, (int*)ctx->eig_d.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev[1]));
+    float tm = 0.0f;
+    HIPCHK(hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]));
+    const double ops = 2.0 * nblk * 8.0 * (double)iters * kCeilAcc * 16384.0;   // 16 x 16 x 64 MACs per MFMA
+    if (ms) *ms = tm;
+    if (tops) *tops = tm > 0.0f ? ops / (tm * 1e-3) / 1e12 : 0.0;
     return HSGEN_OK;
 }
 

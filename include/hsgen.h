@@ -237,6 +237,15 @@ hsgen_status hsgen_create_hs_noalloc(hsgen_ctx* ctx, const hsgen_sys* sys,
 hsgen_status hsgen_eigensolve(hsgen_ctx* ctx, int32_t n, const double* H, const double* S, int64_t ld,
                               double* w, double* V, int64_t ldv, double* ms);
 
+/* Diagnostic, no counterpart in the reference: the sustained rate of the
+ * INT8 MFMA instruction the split-precision GEMMs use, on every CU, with
+ * operands that change from one instruction to the next (random bytes, as
+ * residues are) and nothing else running.  The board's power limit lowers
+ * the clock under this load, so *tops (INT8 TOP/s) is the ceiling those
+ * GEMMs can reach on this board, below the nominal 5.0 POP/s.  iters <= 0:
+ * 300000 iterations (about half a second).  *tops and *ms may be NULL. */
+hsgen_status hsgen_int8_mfma_ceiling(hsgen_ctx* ctx, int32_t iters, double* tops, double* ms);
+
 /* ---------------------------------------------------------------------------
  * System set-up on the host (no GPU needed): what turns a Fleur dump into the
  * hsgen_system above.

@@ -402,6 +402,14 @@ def test_autotune_picks_and_applies_a_tile(ctx):
     assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
 
 
+def test_int8_mfma_ceiling_diagnostic(ctx):
+    """The INT8 MFMA ceiling diagnostic runs on every CU and reports a rate
+    at most the nominal dense peak (5.0 POP/s)."""
+    tops, ms = ctx.int8_mfma_ceiling(20000)
+    print(f"INT8 MFMA ceiling: {tops:.0f} TOP/s in {ms:.1f} ms", flush=True)
+    assert ms > 0 and 0 < tops <= 5000 * 1.02
+
+
 def test_tuning_knobs_keep_the_bits(ctx, monkeypatch):
     """HSGEN_TAPP_TILE (another tile for the T application) and
     HSGEN_RESIDUE_PASSES (batched split batches sized for one residue pass)
