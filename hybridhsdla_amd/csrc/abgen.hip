@@ -169,10 +169,12 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
                 double Bu = -m.u * Knorm * dfj[l] + m.u_dr * fj[l];
                 double cr = p.vol_prefac * pc * m.inv_wronk;
                 double ci = p.vol_prefac * ps * m.inv_wronk;
+                // folded operands: the B rows' factor carries sqrt(||u_dot_l||^2)
+                const double bs = (kCompact && p.fold) ? m.sqrt_un : 1.0;
                 rf[4 * l + 0] = cr * Au;
                 rf[4 * l + 1] = ci * Au;
-                rf[4 * l + 2] = cr * Bu;
-                rf[4 * l + 3] = ci * Bu;
+                rf[4 * l + 2] = cr * Bu * bs;
+                rf[4 * l + 3] = ci * Bu * bs;
             }
         }
         if (kCompact && tid <= na) {
@@ -244,11 +246,8 @@ __global__ __launch_bounds__(256) void k_abgen(AbgenParams p)
                 const double2 y = s_yl[t * L + lm];
                 const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
                 double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
-                if (p.fold) {
-                    const double sc = isB ? p.rad[(int64_t)(a0 + t) * nl + l].sqrt_un : 1.0;
-                    p.Z[base + q] = make_double2(v.x * sc, v.y * sc);
-                } else {
-                    p.Z[base + q] = v;
+                p.Z[base + q] = v;
+                if (!p.fold) {
                     double sc = isB ? p.rad[(int64_t)(a0 + t) * nl + l].dE_u_norm : 1.0;
                     p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
                 }
@@ -366,6 +365,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
     double* s_ph = s_un + chunk * nl;                    // [chunk][2]   cos, sin of the phase 2 pi K.tau
     int* s_off = (int*)(s_ph + chunk * 2);               // [chunk+1]    row prefix (compact)
     int* s_rc = s_off + chunk + 1;                       // [chunk]      rotation class
+    unsigned char* s_l = (unsigned char*)(s_rc + chunk); // [L]          l of lm
 
     if (kCompact && col >= p.ng) {
         const double2 z = make_double2(0.0, 0.0);
@@ -385,6 +385,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
     const double c2 = p.bmat[2] * K0 + p.bmat[5] * K1 + p.bmat[8] * K2;
     const double Knorm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
 
+    for (int lm = tid; lm < L; lm += blockDim.x) s_l[lm] = (unsigned char)l_of_lm(lm);
     // per class: Bessel values (threads 0..nrmt-1), directions (64..64+nrot-1)
     if (tid < p.nrmt) {
         double* fj = s_bes + tid * 2 * nl;
@@ -465,12 +466,14 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
             const double Bu = -m.u * Knorm * dfj[l] + m.u_dr * fj[l];
             const double cr = p.vol_prefac * pc * m.inv_wronk;
             const double ci = p.vol_prefac * ps * m.inv_wronk;
+            // folded operands: the B rows' factor carries sqrt(||u_dot_l||^2)
+            const double bs = (kCompact && p.fold) ? m.sqrt_un : 1.0;
             double* rf = s_rf + (t * nl + l) * 4;
             rf[0] = cr * Au;
             rf[1] = ci * Au;
-            rf[2] = cr * Bu;
-            rf[3] = ci * Bu;
-            s_un[t * nl + l] = p.fold ? m.sqrt_un : m.dE_u_norm;
+            rf[2] = cr * Bu * bs;
+            rf[3] = ci * Bu * bs;
+            s_un[t * nl + l] = m.dE_u_norm;
         }
         __syncthreads();
         if (kCompact) {
@@ -483,15 +486,13 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
                 const int r = q - s_off[t];
                 const bool isB = r >= nlm2;
                 const int lm = isB ? r - nlm2 : r;
-                const int l = l_of_lm(lm);
+                const int l = s_l[lm];
                 const double2 y = s_y[s_rc[t] * L + lm];
                 const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
                 const double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
-                const double sc = isB ? s_un[t * nl + l] : 1.0;
-                if (p.fold) {
-                    p.Z[base + q] = make_double2(v.x * sc, v.y * sc);
-                } else {
-                    p.Z[base + q] = v;
+                p.Z[base + q] = v;
+                if (!p.fold) {
+                    const double sc = isB ? s_un[t * nl + l] : 1.0;
                     p.Zs[base + q] = make_double2(v.x * sc, v.y * sc);
                 }
             }
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
                 const int t = q / L;
                 const int lm = q - t * L;
                 const int64_t idx = (int64_t)col * ld + (int64_t)(a0 + t) * L + lm;
-                const int l = l_of_lm(lm);
+                const int l = s_l[lm];
                 if (l <= p.atoms[a0 + t].lmax) {
                     const double2 y = s_y[s_rc[t] * L + lm];
                     const double* rf = s_rf + (t * nl + l) * 4;
@@ -529,7 +530,7 @@ size_t abgen_cls_lds_bytes(int chunk, int nl, int L, int nrot, int nrmt)
 {
     return 16 * (size_t)nrot * L +
            sizeof(double) * ((size_t)nrmt * 2 * nl + (size_t)nrot * 4 + (size_t)chunk * nl * 5 + (size_t)chunk * 2) +
-           sizeof(int) * (size_t)(2 * chunk + 1);
+           sizeof(int) * (size_t)(2 * chunk + 1) + (size_t)L;
 }
 
 size_t abgen_lds_bytes(int chunk, int nl, int L)
