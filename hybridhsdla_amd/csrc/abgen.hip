@@ -479,15 +479,25 @@ __global__ __launch_bounds__(256) void k_abgen_cls(AbgenParams p)
         if (kCompact) {
             const int nrows = s_off[na];
             const int64_t base = (int64_t)col * p.ldz + 2 * (int64_t)p.atoms[a0].off;
-            int t = 0;   // the atom of row q; q only grows, so t does
+            // the atom t of row q and its first row lo, last row + 1 hi,
+            // rotation class rc, held in registers: q only grows, so t does,
+            // and they are re-read only when q passes hi
+            int t = 0, lo = 0, hi = s_off[1], rc = s_rc[0];
             for (int q = tid; q < nrows; q += blockDim.x) {
-                while (t + 1 < na && s_off[t + 1] <= q) ++t;
-                const int nlm2 = (s_off[t + 1] - s_off[t]) >> 1;
-                const int r = q - s_off[t];
+                if (q >= hi) {
+                    do {
+                        ++t;
+                        lo = hi;
+                        hi = s_off[t + 1];
+                    } while (q >= hi);
+                    rc = s_rc[t];
+                }
+                const int nlm2 = (hi - lo) >> 1;
+                const int r = q - lo;
                 const bool isB = r >= nlm2;
                 const int lm = isB ? r - nlm2 : r;
                 const int l = s_l[lm];
-                const double2 y = s_y[s_rc[t] * L + lm];
+                const double2 y = s_y[rc * L + lm];
                 const double* rf = s_rf + (t * nl + l) * 4 + (isB ? 2 : 0);
                 const double2 v = make_double2(y.x * rf[0] - y.y * rf[1], y.x * rf[1] + y.y * rf[0]);
                 p.Z[base + q] = v;
