@@ -103,18 +103,51 @@ OzPlan oz_plan(int nmod)
 // an element of column i to beta bits of the column's norm exponent e_i leaves
 // an independent error uniform within +-2^(e_i - beta - 1) in each real part,
 // so E|dC_ij|^2 = 2^-2beta / 6 (2^(2 eL_i) ||R_j||^2 + 2^(2 eR_j) ||L_i||^2);
-// the lower triangle holds about half of all (i, j).
-double oz_model(const int32_t* el, const double* sl, const int32_t* er, const double* sr, int ng, int beta,
-                double tril_norm2)
+// the lower triangle holds about half of all (i, j).  The column sums do not
+// depend on beta: they are formed once per product (as multiples of
+// 2^(2 max e), which scales every partial sum by the same power of two) and
+// the model is then evaluated for any beta in O(1).
+struct OzModelSums {
+    int el_max = 0, er_max = 0;
+    double a = 0.0;   // sum_i 2^(2 (eL_i - el_max))
+    double b = 0.0;   // sum_j ||R_j||^2
+    double c = 0.0;   // sum_j 2^(2 (eR_j - er_max))
+    double d = 0.0;   // sum_i ||L_i||^2
+};
+
+// 4^-d for d = 0 .. 537 (below 2^-1074 a power of two rounds to zero)
+const double* pow4_neg()
 {
-    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    static const std::array<double, 538> t = [] {
+        std::array<double, 538> v{};
+        for (int d = 0; d < 538; ++d) v[d] = std::ldexp(1.0, -2 * d);
+        return v;
+    }();
+    return t.data();
+}
+
+OzModelSums oz_model_sums(const int32_t* el, const double* sl, const int32_t* er, const double* sr, int ng)
+{
+    OzModelSums m;
+    if (ng <= 0) return m;
+    const double* p4 = pow4_neg();
+    m.el_max = *std::max_element(el, el + ng);
+    m.er_max = *std::max_element(er, er + ng);
     for (int i = 0; i < ng; ++i) {
-        a += std::ldexp(1.0, 2 * (el[i] - beta));
-        b += sr[i];
-        c += std::ldexp(1.0, 2 * (er[i] - beta));
-        d += sl[i];
+        const int dl = m.el_max - el[i], dr = m.er_max - er[i];
+        m.a += dl < 538 ? p4[dl] : 0.0;
+        m.b += sr[i];
+        m.c += dr < 538 ? p4[dr] : 0.0;
+        m.d += sl[i];
     }
-    return tril_norm2 > 0.0 ? std::sqrt((a * b + c * d) / 12.0 / tril_norm2) : 0.0;
+    return m;
+}
+
+double oz_model(const OzModelSums& m, int beta, double tril_norm2)
+{
+    const double a = std::ldexp(m.a, 2 * (m.el_max - beta));
+    const double c = std::ldexp(m.c, 2 * (m.er_max - beta));
+    return tril_norm2 > 0.0 ? std::sqrt((a * m.b + c * m.d) / 12.0 / tril_norm2) : 0.0;
 }
 
 }  // namespace
@@ -611,12 +644,16 @@ hsgen_status run_ozaki(hsgen_ctx* ctx, const Geometry& geo, const std::vector<Oz
             tril2_smp[k][w] = 0.5 * (y + dg) * (double)ks[k].ng / (double)nb[k];
         }
     }
-    auto model = [&](int k, int which, int beta, double tril2) {   // which 0: S (R = Zs), 1: H (R = W)
-        const size_t o = (which == 0 ? 1 : 2) * (size_t)gp;
-        const size_t c = (size_t)ks[k].col0;
-        return oz_model(he.data() + c, hss.data() + c, he.data() + o + c, hss.data() + o + c, ks[k].ng, beta,
-                        tril2);
-    };
+    // the model's column sums per k-point and product (which 0: S, R = Zs; 1: H, R = W)
+    std::vector<std::array<OzModelSums, 2>> msums(nk);
+    for (int k = 0; k < nk; ++k)
+        for (int which = 0; which < 2; ++which) {
+            const size_t o = (which == 0 ? 1 : 2) * (size_t)gp;
+            const size_t c = (size_t)ks[k].col0;
+            msums[k][which] =
+                oz_model_sums(he.data() + c, hss.data() + c, he.data() + o + c, hss.data() + o + c, ks[k].ng);
+        }
+    auto model = [&](int k, int which, int beta, double tril2) { return oz_model(msums[k][which], beta, tril2); };
     auto worst_model = [&](int k, int beta, const std::array<double, 2>& tril2) {
         return std::max(do_s ? model(k, 0, beta, tril2[0]) : 0.0, do_h ? model(k, 1, beta, tril2[1]) : 0.0);
     };
