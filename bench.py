@@ -31,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "sec per H+S & TFLOP/s: FP64 MFMA vs split-precision at ≤1e-12 rel err"
 FP64_PEAK = 78.6      # TFLOP/s, MI355X dense FP64 matrix
 INT8_PEAK = 5000.0    # TOP/s, MI355X dense INT8
+BF16_PEAK = 2500.0    # TFLOP/s, MI355X dense BF16
 HBM_PEAK = 8000.0     # GB/s
 
 WORKLOAD = {
@@ -216,22 +217,25 @@ def lower_rel_err(X, ref, rows=2048):
     return (num / den) ** 0.5 if den > 0 else 0.0
 
 
-def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, warmup, idx, ceil=None):
-    """Split precision (INT8 Ozaki) on the same k-point at the error target,
-    timed like the FP64 run, with its error against the FP64 result on the
-    whole lower triangle; returns the entry and the result's principal
-    submatrices on the stratified sample idx."""
+def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, warmup, idx, ceil=None,
+              bf16=False):
+    """Split precision (Ozaki scheme II on INT8 MFMA, or with bf16 on BF16
+    MFMA) on the same k-point at the error target, timed like the FP64 run,
+    with its error against the FP64 result on the whole lower triangle;
+    returns the entry and the result's principal submatrices on the
+    stratified sample idx."""
     import numpy as np
     import torch
-    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8
 
+    prec = HSGEN_PREC_OZAKI_BF16 if bf16 else HSGEN_PREC_OZAKI_INT8
     for _ in range(warmup):
-        step(HSGEN_PREC_OZAKI_INT8, target)
+        step(prec, target)
     torch.cuda.synchronize()
     reps = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        reps.append(step(HSGEN_PREC_OZAKI_INT8, target))
+        reps.append(step(prec, target))
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     eH = lower_rel_err(H, H64)
@@ -240,8 +244,9 @@ def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, 
     r = reps[-1]
     nmod = int(r["slices_used"])
     ms_ig = float(np.mean([x["ms_igemm"] for x in reps]))
-    ops = nmod * 16.0 * G * G * system.kdim      # INT8 ops of the nmod modulus GEMMs
+    ops = nmod * 16.0 * G * G * system.kdim      # INT8 ops (BF16 flops) of the nmod modulus GEMMs
     ach = ops / (ms_ig * 1e-3) / 1e12 if ms_ig > 0 else None
+    pk = BF16_PEAK if bf16 else INT8_PEAK
     entry = {
         "target_rel_err": target,
         "moduli": nmod,
@@ -255,17 +260,21 @@ def split_run(args, system, G, H, S, H64, S64, step, fp64_ms, F, target, steps, 
         "err_model_H": r["err_est_h"], "err_model_S": r["err_est_s"],
         "err_sample_H": r["err_smp_h"], "err_sample_S": r["err_smp_s"],
         "breakdown_ms": {"contract": round(float(np.mean([x["ms_contract"] for x in reps])), 3),
-                         "int8_gemm": round(ms_ig, 3)},
+                         "bf16_gemm" if bf16 else "int8_gemm": round(ms_ig, 3)},
         "roofline": {
-            "bound": "mfma", "kernel": "k_igemm2_crt (INT8 residue GEMMs, one launch per modulus)",
-            "achieved": round(ach, 1) if ach else None, "peak": INT8_PEAK, "unit": "TOP/s",
-            "frac": round(ach / INT8_PEAK, 4) if ach else None,
-            "traffic": (load_traffic("igemm2_crt", f"cfg{args.config}") or {}).get("bytes_per_launch"),
+            "bound": "mfma",
+            "kernel": f"k_igemm2_crt ({'BF16' if bf16 else 'INT8'} residue GEMMs, one launch per modulus)",
+            "achieved": round(ach, 1) if ach else None, "peak": pk, "unit": "TFLOP/s" if bf16 else "TOP/s",
+            "frac": round(ach / pk, 4) if ach else None,
+            "traffic": None if bf16 else (load_traffic("igemm2_crt", f"cfg{args.config}") or {}).get("bytes_per_launch"),
             "algorithmic_ops_per_launch": 16.0 * G * G * system.kdim,
             "avg_launch_ms": round(ms_ig / max(nmod, 1), 3),
         },
     }
-    with_ceiling(entry["roofline"], ceil)
+    if bf16:
+        entry["mfma"] = "bf16"
+    else:
+        with_ceiling(entry["roofline"], ceil)
     return entry, smp
 
 
@@ -276,7 +285,7 @@ def bench_batched(args) -> int:
     import numpy as np
     import torch
     from hybridhsdla_amd import make_config
-    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8
 
     t0 = time.perf_counter()
     system, kps = make_config(4)
@@ -290,7 +299,7 @@ def bench_batched(args) -> int:
     mats = [(torch.zeros((n, n), dtype=torch.complex128, device=dev),
              torch.zeros((n, n), dtype=torch.complex128, device=dev)) for n in ngs]
     outs = [(a.data_ptr(), b.data_ptr(), n) for (a, b), n in zip(mats, ngs)]
-    prec = HSGEN_PREC_FP64 if args.precision == "fp64" else HSGEN_PREC_OZAKI_INT8
+    prec = {"fp64": HSGEN_PREC_FP64, "int8": HSGEN_PREC_OZAKI_INT8, "bf16": HSGEN_PREC_OZAKI_BF16}[args.precision]
 
     def step():
         return ctx.create_HS2_batched(sysh, kps, precision=prec, target_rel_err=args.split_target,
@@ -323,13 +332,15 @@ def bench_batched(args) -> int:
         ops = sum(r["slices_used"] * 16.0 * r["ng"] ** 2 * system.kdim for r in reps[-1])
         ms_ig = float(np.mean([sum(x["ms_igemm"] for x in r) for r in reps]))
         ach = ops / (ms_ig * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_igemm2_crt (INT8 residue GEMMs, one launch per modulus over "
-                "the tiles of every k-point of a memory batch)",
-                "achieved": round(ach, 1), "peak": INT8_PEAK, "unit": "TOP/s", "frac": round(ach / INT8_PEAK, 4),
-                "traffic": None, "ms_igemm_per_step": round(ms_ig, 2),
+        bf = prec == HSGEN_PREC_OZAKI_BF16
+        pk = BF16_PEAK if bf else INT8_PEAK
+        roof = {"bound": "mfma", "kernel": f"k_igemm2_crt ({'BF16' if bf else 'INT8'} residue GEMMs, one launch "
+                "per modulus over the tiles of every k-point of a memory batch)",
+                "achieved": round(ach, 1), "peak": pk, "unit": "TFLOP/s" if bf else "TOP/s",
+                "frac": round(ach / pk, 4), "traffic": None, "ms_igemm_per_step": round(ms_ig, 2),
                 "moduli": sorted({int(x["slices_used"]) for x in reps[-1]})}
-        ceil = int8_ceiling(ctx)
-        with_ceiling(roof, ceil)
+        if not bf:
+            with_ceiling(roof, int8_ceiling(ctx))
     check = None
     if not args.no_check:
         try:
@@ -362,7 +373,8 @@ def bench_batched(args) -> int:
         "metric": METRIC, "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "sec_per_hs": round(ms_step * 1e-3 / len(kps), 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if prec == HSGEN_PREC_FP64 else "int8-slices/f64",
+        "vs_baseline": None, "dtype": {HSGEN_PREC_FP64: "f64", HSGEN_PREC_OZAKI_INT8: "int8-slices/f64",
+                                       HSGEN_PREC_OZAKI_BF16: "bf16-slices/f64"}[prec],
         "data": "synthetic (seeded system generator; random-init T-matrices and radial values)",
         "config": {"workload": WORKLOAD[4] + f", N_G {min(ngs)}..{max(ngs)}, K={system.kdim}",
                    "natoms": system.natoms, "n_k": len(kps), "n_g_total": int(sum(ngs)),
@@ -419,8 +431,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1, help="untimed H+S evaluations before the timed ones")
     ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 4],
                     help="workload; 4 = 64 k-points through the batched entry point")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "int8"],
-                    help="arithmetic of the measured run (int8: split precision)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "int8", "bf16"],
+                    help="arithmetic of the measured run (int8, bf16: split precision on INT8 or BF16 MFMA)")
     ap.add_argument("--slices", type=int, default=0, help="split precision: modulus count (0: from the error model)")
     ap.add_argument("--tile", type=int, default=0, help="tile configuration id (0: default or autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -431,6 +443,8 @@ def main(argv=None) -> int:
                     help="skip the split-precision comparison run after the FP64 run")
     ap.add_argument("--split-target", type=float, default=1e-12,
                     help="split precision: relative error target of the headline run")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="skip the split-precision run on BF16 MFMA at --split-target")
     ap.add_argument("--split-sweep", default="1e-10,1e-12,1e-13",
                     help="split precision: further targets, timed with at most 3 steps each")
     ap.add_argument("--eig", action="store_true",
@@ -449,7 +463,7 @@ def main(argv=None) -> int:
     import numpy as np
     import torch
     from hybridhsdla_amd import make_config
-    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8
 
     if args.config == 4:
         return bench_batched(args)
@@ -467,7 +481,7 @@ def main(argv=None) -> int:
     sysh = ctx.upload(system)
     H = torch.zeros((G, G), dtype=torch.complex128, device=dev)
     S = torch.zeros((G, G), dtype=torch.complex128, device=dev)
-    prec = HSGEN_PREC_FP64 if args.precision == "fp64" else HSGEN_PREC_OZAKI_INT8
+    prec = {"fp64": HSGEN_PREC_FP64, "int8": HSGEN_PREC_OZAKI_INT8, "bf16": HSGEN_PREC_OZAKI_BF16}[args.precision]
     if args.copy_probe > 0:
         n = int(args.copy_probe * 2**30) // 8
         a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
@@ -522,18 +536,22 @@ def main(argv=None) -> int:
         "avg_launch_ms": round(ms_contract, 3),
     }
     if prec != HSGEN_PREC_FP64:
-        # split precision as the main mode: the INT8 residue GEMMs dominate
+        # split precision as the main mode: the residue GEMMs dominate
         nmod = int(reps[-1]["slices_used"])
         ms_ig = float(np.mean([r["ms_igemm"] for r in reps]))
         ach = nmod * f_contract / (ms_ig * 1e-3) / 1e12
+        bf = prec == HSGEN_PREC_OZAKI_BF16
+        pk = BF16_PEAK if bf else INT8_PEAK
         roofline = {
-            "bound": "mfma", "kernel": f"k_igemm2_crt tile {tile_used} (INT8 residue GEMMs, one launch per modulus)",
-            "achieved": round(ach, 1), "peak": INT8_PEAK, "unit": "TOP/s", "frac": round(ach / INT8_PEAK, 4),
-            "traffic": (load_traffic("igemm2_crt", workload_tag) or {}).get("bytes_per_launch"),
+            "bound": "mfma",
+            "kernel": f"k_igemm2_crt tile {tile_used} ({'BF16' if bf else 'INT8'} residue GEMMs, one launch per modulus)",
+            "achieved": round(ach, 1), "peak": pk, "unit": "TFLOP/s" if bf else "TOP/s", "frac": round(ach / pk, 4),
+            "traffic": None if bf else (load_traffic("igemm2_crt", workload_tag) or {}).get("bytes_per_launch"),
             "algorithmic_ops_per_launch": f_contract, "avg_launch_ms": round(ms_ig / nmod, 3),
         }
-    ceil = int8_ceiling(ctx) if (prec != HSGEN_PREC_FP64 or not args.no_split) else None
-    if prec != HSGEN_PREC_FP64:
+    ceil = int8_ceiling(ctx) if (prec == HSGEN_PREC_OZAKI_INT8 or (prec == HSGEN_PREC_FP64 and not args.no_split)) \
+        else None
+    if prec == HSGEN_PREC_OZAKI_INT8:
         with_ceiling(roofline, ceil)
     abgen_bytes = float(reps[-1]["bytes_abgen"])
     abgen = {
@@ -580,6 +598,7 @@ def main(argv=None) -> int:
     samples = {"fp64": (idx, *gather_sample(H, S, idx))} if prec == HSGEN_PREC_FP64 else {}
     split = None
     sweep = None
+    split_bf16 = None
     if prec == HSGEN_PREC_FP64 and not args.no_split:
         try:
             H64 = H.clone()
@@ -600,6 +619,18 @@ def main(argv=None) -> int:
                 log(f"[bench] split precision {target:.0e}: {entry['moduli']} moduli, {entry['sec_per_hs']} s, "
                     f"{entry['speedup_vs_fp64']}x, err H {entry['rel_err_H_vs_fp64']:.2e} S "
                     f"{entry['rel_err_S_vs_fp64']:.2e}")
+            if not args.no_bf16:
+                # the same slices on BF16 MFMA: same moduli, same bits as INT8
+                target = args.split_target
+                entry, smp = split_run(args, system, G, H, S, H64, S64, step, ms_step, F, target,
+                                       min(args.steps, 3), min(args.warmup, 1), idx, bf16=True)
+                ref = samples.get(f"split_{target:.0e}")
+                entry["sample_bitwise_equal_to_int8"] = bool(
+                    ref is not None and np.array_equal(smp[0], ref[1]) and np.array_equal(smp[1], ref[2]))
+                split_bf16 = entry
+                log(f"[bench] split precision on BF16 MFMA {target:.0e}: {entry['moduli']} moduli, "
+                    f"{entry['sec_per_hs']} s, {entry['speedup_vs_fp64']}x, same bits as INT8 on the sample: "
+                    f"{entry['sample_bitwise_equal_to_int8']}")
             del H64, S64
         except Exception as e:
             split = {"error": str(e)}
@@ -627,7 +658,8 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == HSGEN_PREC_FP64 else "int8-slices/f64",
+        "dtype": {HSGEN_PREC_FP64: "f64", HSGEN_PREC_OZAKI_INT8: "int8-slices/f64",
+                  HSGEN_PREC_OZAKI_BF16: "bf16-slices/f64"}[prec],
         "data": "synthetic (seeded system generator; random-init T-matrices and radial values)",
         "config": {
             "workload": WORKLOAD[args.config] + f", N_G={G} (Kmax={kp.kmax:.4f}), K={system.kdim}",
@@ -642,6 +674,7 @@ def main(argv=None) -> int:
         "self_check": check,
         "split_precision": split,
         "split_sweep": sweep,
+        "split_bf16": split_bf16,
         "int8_mfma_ceiling": ceil,
         "host_outputs": host_out,
         "eigensolve": eig,

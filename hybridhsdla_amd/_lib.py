@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhsgen.so"
 HSGEN_OK = 0
 HSGEN_PREC_FP64 = 0
 HSGEN_PREC_OZAKI_INT8 = 1
+HSGEN_PREC_OZAKI_BF16 = 2
 HSGEN_FILL_UPPER = 1
 HSGEN_SKIP_H = 2
 HSGEN_SKIP_S = 4

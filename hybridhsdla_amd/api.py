@@ -18,12 +18,12 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import (HSGEN_FILL_UPPER, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_INT8, HSGEN_SKIP_H,
+from ._lib import (HSGEN_FILL_UPPER, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8, HSGEN_SKIP_H,
                    HSGEN_SKIP_S, HsgenError, HsgenKpoint, HsgenOpts, HsgenReport, HsgenSystem)
 from .system import KPoint, System
 
 __all__ = ["Context", "SystemHandle", "HsgenError", "HSGEN_FILL_UPPER", "HSGEN_SKIP_H",
-           "HSGEN_SKIP_S", "HSGEN_PREC_FP64", "HSGEN_PREC_OZAKI_INT8"]
+           "HSGEN_SKIP_S", "HSGEN_PREC_FP64", "HSGEN_PREC_OZAKI_INT8", "HSGEN_PREC_OZAKI_BF16"]
 
 
 def _ptr(a: np.ndarray) -> int:

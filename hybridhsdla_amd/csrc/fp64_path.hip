@@ -23,9 +23,19 @@ namespace hsgen {
 
 // column padding for the split-precision path: a multiple of both sides of
 // the INT8 tile (the same operand supplies rows and columns of S and H)
-int64_t int8_col_align(int tile)
+int split_cfg(int precision, int tile)
 {
-    const IgemmLaunchCfg c = igemm_cfg((tile > 0 && igemm_cfg(tile).bm > 0) ? tile : kIgemmCfg);
+    const int es = precision == HSGEN_PREC_OZAKI_BF16 ? 2 : 1;
+    if (tile > 0) {
+        const IgemmLaunchCfg c = igemm_cfg(tile);
+        if (c.bm > 0 && c.esize == es) return tile;
+    }
+    return es == 2 ? kBf16Cfg : kIgemmCfg;
+}
+
+int64_t split_col_align(int precision, int tile)
+{
+    const IgemmLaunchCfg c = igemm_cfg(split_cfg(precision, tile));
     return std::lcm<int64_t>(c.bm, c.bn);
 }
 
@@ -267,8 +277,8 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
     work.clear();
     int icfg = 0;
     OzResult oz;
-    if (o.precision == HSGEN_PREC_OZAKI_INT8) {
-        icfg = (o.tile > 0 && igemm_cfg(o.tile).bm > 0) ? o.tile : kIgemmCfg;
+    if (is_split(o.precision)) {
+        icfg = split_cfg(o.precision, o.tile);
         std::vector<OzK> ks(1);
         ks[0].ng = ng;
         ks[0].ncols = geo.gp;
@@ -322,7 +332,7 @@ hsgen_status finish_hs(hsgen_ctx* ctx, hsgen_sys* s, const Geometry& geo, int ng
         rep->ms_igemm = oz.ms_igemm;
         rep->err_smp_h = oz.smp_h;
         rep->err_smp_s = oz.smp_s;
-        rep->tile_used = o.precision == HSGEN_PREC_OZAKI_INT8 ? icfg : geo.cfg;
+        rep->tile_used = is_split(o.precision) ? icfg : geo.cfg;
     }
     return HSGEN_OK;
 }
@@ -630,8 +640,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (st) return st;
     hsgen_opts o{};
     if (opts) o = *opts;
-    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
-        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    if (!valid_precision(o.precision)) return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
     if (o.slices < 0 || o.slices > kMaxModuli)
         return fail(ctx, HSGEN_ERR_ARG, "slices %d outside [0, %d]", o.slices, kMaxModuli);
     const int ng = kp->ng;
@@ -643,7 +652,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
     Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
-                            o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
+                            o.precision == HSGEN_PREC_FP64 ? 1 : split_col_align(o.precision, o.tile));
     geo.fold = s->fold;
     st = upload_g(ctx, kp);
     if (st) return st;
@@ -684,8 +693,7 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32
         return fail(ctx, HSGEN_ERR_ARG, "null argument");
     hsgen_opts o{};
     if (opts) o = *opts;
-    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
-        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    if (!valid_precision(o.precision)) return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
     for (int i = 0; i < nk; ++i) {
         hsgen_status st = check_kpoint(ctx, &kps[i]);
         if (st) return st;
@@ -726,8 +734,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
         return fail(ctx, HSGEN_ERR_ARG, "max_lmax %d inconsistent with L %d", max_lmax, L);
     hsgen_opts o{};
     if (opts) o = *opts;
-    if (o.precision != HSGEN_PREC_FP64 && o.precision != HSGEN_PREC_OZAKI_INT8)
-        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
+    if (!valid_precision(o.precision)) return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", o.precision);
     if (o.slices < 0 || o.slices > kMaxModuli)
         return fail(ctx, HSGEN_ERR_ARG, "slices %d outside [0, %d]", o.slices, kMaxModuli);
     if (ld < std::max(G, 1)) return fail(ctx, HSGEN_ERR_ARG, "ld < G");
@@ -771,7 +778,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     if (G == 0) return HSGEN_OK;
     o.tile = pick_tile(ctx, o, G, s.kdim);
     Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
-                            o.precision == HSGEN_PREC_FP64 ? 1 : int8_col_align(o.tile));
+                            o.precision == HSGEN_PREC_FP64 ? 1 : split_col_align(o.precision, o.tile));
     geo.fold = s.fold;
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, s.fold ? s.h_sqrtn.data() : nullptr, geo.t.bk,
                                geo.t.bm);

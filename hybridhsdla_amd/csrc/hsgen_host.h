@@ -37,6 +37,7 @@ namespace hsgen {
 constexpr int kDefaultTile = 11;
 constexpr int kFallbackTile = 8;   // register-staged, 64-bit addressing
 constexpr int kIgemmCfg = 51;      // default INT8 kernel configuration (256 x 192 tiles)
+constexpr int kBf16Cfg = 61;       // default BF16 kernel configuration (256 x 192 tiles)
 constexpr int kSampleTile = 12;    // FP64 tile of the split-precision sample: 64 x 64, so 64 columns
 constexpr size_t kMaxOzGraphs = 16; // cached split-precision launch-segment sets per context
 
@@ -171,7 +172,17 @@ inline int n_operands(const Geometry& g) { return g.fold ? 2 : 3; }
 // The right operand of S: Zs, or Y itself (in ctx->Z) when folded.
 inline double2* zs_ptr(hsgen_ctx* ctx, const Geometry& g) { return (double2*)(g.fold ? ctx->Z.p : ctx->Zs.p); }
 
-int64_t int8_col_align(int tile);
+// Split precision (INT8 or BF16 MFMA on the same residues).
+inline bool is_split(int precision)
+{
+    return precision == HSGEN_PREC_OZAKI_INT8 || precision == HSGEN_PREC_OZAKI_BF16;
+}
+inline bool valid_precision(int precision) { return precision == HSGEN_PREC_FP64 || is_split(precision); }
+// The split-precision kernel configuration of a call: opts.tile when it names
+// one for the precision's MFMA type, else the precision's default.
+int split_cfg(int precision, int tile);
+// Column padding of split-precision operands: a multiple of both tile sides.
+int64_t split_col_align(int precision, int tile);
 Geometry geometry(int tile, int kdim, int ng, int64_t col_align = 1);
 void add_hs_problems(std::vector<GemmProblem>& probs, std::vector<TileWork>& work, const Geometry& geo, int ng,
                      const double2* Z, const double2* Zs, const double2* W, double2* H, double2* S, int64_t ld,

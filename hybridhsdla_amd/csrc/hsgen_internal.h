@@ -130,6 +130,7 @@ struct OzResParams {
     int8_t* out_rot[kMaxModuli]; // rotated pairs or null
     int32_t rot_kind;            // 1: (-im, re) (left operand), 2: (im, -re) (right operand)
     int64_t ldo;
+    int32_t esize;               // bytes per real residue: 1 (INT8) or 2 (bf16)
 };
 
 struct IgemmProblem {
@@ -170,6 +171,7 @@ struct IgemmLaunchCfg {
     int needs_rot;              // 1: reads precomputed (-Li, Lr); 2: reads precomputed (Ri, -Rr)
     int max_kchunk;             // > 0: no in-kernel mod reduction, launch at most this depth (complex)
     int wgm, wgn;               // waves along rows / columns of the tile
+    int esize;                  // bytes per real residue operand: 1 (INT8 MFMA) or 2 (BF16 MFMA)
 };
 
 struct OzFinalParams {

@@ -1,33 +1,41 @@
 // Split-precision contraction on INT8 MFMA (Ozaki scheme II: integer
 // residues modulo small pairwise-coprime moduli, exact INT32 products,
-// Chinese-remainder recombination in FP64).
+// Chinese-remainder recombination in FP64), or on BF16 MFMA with the same
+// residues held as bf16 and exact FP32 accumulation.
 //
 // For C = L^H R (complex, depth kd):
 //   1. per column j of each FP64 operand X: e_j with ||X_:j||_2 < 2^e_j,
 //      X'_kj = rint(X_kj 2^(beta - e_j)), so |X'| < 2^beta and, rounding
 //      included, ||X'_:j||_2 < 2^beta (1 + 2^-20) for beta >= 19 + log2(2 kd) / 2;
 //   2. for each modulus p_m (<= 255): the symmetric residues X' mod p_m fit
-//      INT8; L is stored twice, as (Lr, Li) and as (-Li, Lr) pairs, so that
+//      INT8 (and bf16 exactly); with the (re, im) pairs interleaved,
 //        Re C' = sum (Lr Rr + Li Ri),   Im C' = sum (-Li Rr + Lr Ri)
-//      are each one INT8 dot product over the interleaved pairs;
-//   3. v_mfma_i32_16x16x64_i8 accumulates exactly in INT32; the accumulator is
-//      reduced mod p_m every <= 2080 MFMAs (|products| <= 127^2) so it never
-//      overflows;
+//      are each one dot product over the pairs, the second with (-Li, Lr)
+//      formed in registers from L;
+//   3. v_mfma_i32_16x16x64_i8 accumulates exactly in INT32, reduced mod p_m
+//      every <= 2080 MFMAs (|products| <= 127^2) so it never overflows;
+//      v_mfma_f32_16x16x32_bf16 accumulates exactly in FP32 while |acc| < 2^24,
+//      reduced mod p_m every 32 MFMAs (32 x 32 x 127^2 + 254 < 2^24);
 //   4. the epilogue stores r_m = C' mod p_m as one byte per real part, in the
 //      order the MFMA fragments hold the tile (coalesced 128-byte stores);
 //   5. the final pass sums r_m * (u_m / p_m) over the moduli (u_m =
-//      (P/p_m)^-1 mod p_m, P the product of the moduli) in double-double,
-//      reduced mod 1: C'/P modulo 1, exact up to the double-double rounding;
-//      it takes the centred fractional part, multiplies by P and rescales by
-//      2^(e_i + e_j - 2 beta).
-// By Cauchy-Schwarz |C'_ij| <= ||L'_:i|| ||R'_:j|| < 2^(2 beta) (1 + 2^-19), so
-// P > 2^(2 beta + 1.01) suffices whatever the depth; the host sets beta from
-// the number of moduli (beta <= 52 keeps every X' an exact double).
+//      (P/p_m)^-1 mod p_m, P the product of the moduli) with each weight split
+//      into parts whose products and sums are exact, takes the centred
+//      fractional part, multiplies by P and rescales by 2^(e_i + e_j - 2 beta).
+// Both MFMA types see the same residues, so their H and S are bit for bit
+// the same.  By Cauchy-Schwarz |C'_ij| <= ||L'_:i|| ||R'_:j|| < 2^(2 beta)
+// (1 + 2^-19), so P > 2^(2 beta + 1.01) suffices whatever the depth; the host
+// sets beta from the number of moduli (beta <= 52 keeps every X' an exact
+// double).
+#include <type_traits>
+
 #include "hsgen_internal.h"
 
 namespace hsgen {
 
 typedef int i4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(1))) const i4v* gi4ptr;
 
 // ---------------------------------------------------------------------------
@@ -226,8 +234,17 @@ __device__ inline uint32_t pack_bytes(int a, int b, int c, int d)
     return ab | cd;
 }
 
+// bf16 bits of an integer |r| <= 256 (exact: 8 significant bits)
+__device__ inline uint32_t bf16_bits(int r)
+{
+    return __float_as_uint((float)r) >> 16;
+}
+
 // One lane per 4 consecutive rows of a column: 4 complex values in, per
-// modulus one 8-byte store of their 4 residue pairs (rows is a multiple of 4).
+// modulus one store of their 4 residue pairs (rows is a multiple of 4): 8
+// bytes of INT8 pairs (ES = 1) or 16 bytes of bf16 pairs (ES = 2, no rotated
+// copies).
+template <int ES>
 __global__ __launch_bounds__(256) void k_residues(OzResParams p)
 {
     const int col = blockIdx.y;
@@ -250,28 +267,43 @@ __global__ __launch_bounds__(256) void k_residues(OzResParams p)
             rr[k] = sym_res(hr[k], lr[k], c, pm, ip);
             ri[k] = sym_res(hi_[k], li[k], c, pm, ip);
         }
-        // bytes (re, im) of row r0 + k at byte offset 2 k
-        if (p.out[m])
-            *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) =
-                make_uint2(pack_bytes(rr[0], ri[0], rr[1], ri[1]), pack_bytes(rr[2], ri[2], rr[3], ri[3]));
-        if (p.out_rot[m]) {
-            int a[4], b[4];
+        if constexpr (ES == 2) {
+            // bf16 (re, im) of row r0 + k at byte offset 4 k
+            if (p.out[m])
+                *(uint4*)(p.out[m] + (int64_t)col * p.ldo + 4 * r0) =
+                    make_uint4(bf16_bits(rr[0]) | bf16_bits(ri[0]) << 16, bf16_bits(rr[1]) | bf16_bits(ri[1]) << 16,
+                               bf16_bits(rr[2]) | bf16_bits(ri[2]) << 16, bf16_bits(rr[3]) | bf16_bits(ri[3]) << 16);
+        } else {
+            // bytes (re, im) of row r0 + k at byte offset 2 k
+            if (p.out[m])
+                *(uint2*)(p.out[m] + (int64_t)col * p.ldo + 2 * r0) =
+                    make_uint2(pack_bytes(rr[0], ri[0], rr[1], ri[1]), pack_bytes(rr[2], ri[2], rr[3], ri[3]));
+            if (p.out_rot[m]) {
+                int a[4], b[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                a[k] = p.rot_kind == 1 ? -ri[k] : ri[k];
-                b[k] = p.rot_kind == 1 ? rr[k] : -rr[k];
+                for (int k = 0; k < 4; ++k) {
+                    a[k] = p.rot_kind == 1 ? -ri[k] : ri[k];
+                    b[k] = p.rot_kind == 1 ? rr[k] : -rr[k];
+                }
+                *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) =
+                    make_uint2(pack_bytes(a[0], b[0], a[1], b[1]), pack_bytes(a[2], b[2], a[3], b[3]));
             }
-            *(uint2*)(p.out_rot[m] + (int64_t)col * p.ldo + 2 * r0) =
-                make_uint2(pack_bytes(a[0], b[0], a[1], b[1]), pack_bytes(a[2], b[2], a[3], b[3]));
         }
     }
 }
 
 hipError_t launch_residues(const OzResParams& p, int ncols, hipStream_t st)
 {
-    if (p.rows % 4 != 0 || p.ldo % 8 != 0) return hipErrorInvalidValue;
+    if (p.esize != 1 && p.esize != 2) return hipErrorInvalidValue;
+    if (p.rows % 4 != 0 || p.ldo % (8 * p.esize) != 0) return hipErrorInvalidValue;
     dim3 grid((p.rows / 4 + 255) / 256, ncols), block(256);
-    hipLaunchKernelGGL(k_residues, grid, block, 0, st, p);
+    if (p.esize == 2) {
+        for (int m = 0; m < p.nmod; ++m)
+            if (p.out_rot[m]) return hipErrorInvalidValue;   // bf16 kernels rotate in registers
+        hipLaunchKernelGGL(k_residues<2>, grid, block, 0, st, p);
+    } else {
+        hipLaunchKernelGGL(k_residues<1>, grid, block, 0, st, p);
+    }
     return hipGetLastError();
 }
 
@@ -477,7 +509,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm_crt(const IgemmProblem
 // q ^ f(c), f(c) = (c / (16 / CPC)) % CPC, which makes the 16-lane fragment
 // reads (16 consecutive columns, one chunk) hit 16 distinct bank groups.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR>
+template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR, int ES = 1>
 struct Igemm2Cfg {
     static constexpr int kThreads = WGM * WGN * 64;
     static constexpr int kWaves = WGM * WGN;
@@ -485,14 +517,16 @@ struct Igemm2Cfg {
     static constexpr int kWN = BN / WGN;
     static constexpr int kRB = kWM / 16;
     static constexpr int kCB = kWN / 16;
-    static constexpr int kCPC = 2 * KS / 16;                     // 16-byte chunks per column
-    static constexpr int kColBytes = 2 * KS;
+    static constexpr int kColBytes = 2 * KS * ES;                // KS complex residues per column
+    static constexpr int kCPC = kColBytes / 16;                  // 16-byte chunks per column
+    static constexpr int kKSteps = kColBytes / 64;               // MFMA k-steps (64 bytes of k) per stage
     static constexpr int kCols = BM + ((ROTR & 1) ? 2 : 1) * BN;  // L, R [, Rrot] columns
     static constexpr int kStageBytes = kCols * kColBytes;
     static constexpr int kWaveLoads = kCols * kCPC / 64;          // 1 KiB wave loads per stage
     static constexpr int kLoadsPerWave = kWaveLoads / kWaves;
     static constexpr int kLdsBytes = NS * kStageBytes;
-    static_assert(KS == 32 || KS == 64, "stage depth: one or two MFMA k-steps");
+    static_assert(KS == 32 || KS == 64, "stage depth: 32 or 64 complex");
+    static_assert(ES == 1 || (ES == 2 && KS == 32 && ROTR == 0), "bf16: 32 complex per stage, rotation in registers");
     static_assert(kWaveLoads % kWaves == 0, "stage must split evenly over waves");
     static_assert(kLoadsPerWave * (NS - 2) <= 63, "vmcnt range");
 };
@@ -536,12 +570,25 @@ __device__ inline int rot_pairs(int x)
     return d;
 }
 
-template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR>
+// (bf16 lanes: (Lr, Li) in the low and high halves of a dword) -> (-Li, Lr):
+// swap the halves, then flip the sign of the new low half
+__device__ inline int rot_bf16(int x)
+{
+    return (int)(__builtin_amdgcn_alignbit((uint32_t)x, (uint32_t)x, 16) ^ 0x8000u);
+}
+
+// ES = 1: INT8 residues on v_mfma_i32_16x16x64_i8; ES = 2: the same residues
+// as bf16 on v_mfma_f32_16x16x32_bf16, FP32 accumulators (exact below 2^24,
+// reduced mod p every md.chunk_stages stages).  A fragment is 16 bytes of
+// one row either way (16 INT8 or 8 bf16 values of k), so the LDS layout,
+// loads and fragment reads are shared; a stage holds KS complex.
+template <int BM, int BN, int KS, int WGM, int WGN, int NS, int ROTR, int ES = 1>
 __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProblem* __restrict__ probs,
                                                                const TileWork* __restrict__ work, int nwork,
                                                                IgemmModulus md)
 {
-    using Cfg = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>;
+    using Cfg = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR, ES>;
+    using AccT = typename std::conditional<ES == 1, i4v, f4v>::type;
     constexpr int RB = Cfg::kRB, CB = Cfg::kCB;
     constexpr int CPC = Cfg::kCPC;
     constexpr int LPW = Cfg::kLoadsPerWave;
@@ -593,7 +640,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
 #pragma unroll
         for (int r = 0; r < LPW; ++r) {
             const __attribute__((address_space(1))) char* gb =
-                (const __attribute__((address_space(1))) char*)(base[r] + (int64_t)stage_k * (2 * KS));
+                (const __attribute__((address_space(1))) char*)(base[r] + (int64_t)stage_k * Cfg::kColBytes);
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(gb + off[r]),
                                              (__attribute__((address_space(3))) void*)(lbase + buf * SB +
                                                                                        (wave * LPW + r) * 1024),
@@ -601,13 +648,13 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
         }
     };
 
-    i4v are[RB][CB], aim[RB][CB];
+    AccT are[RB][CB], aim[RB][CB];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-            are[i][j] = i4v{0, 0, 0, 0};
-            aim[i][j] = i4v{0, 0, 0, 0};
+            are[i][j] = AccT{0, 0, 0, 0};
+            aim[i][j] = AccT{0, 0, 0, 0};
         }
 
     const int nst = pb.kdepth / KS;
@@ -631,7 +678,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     const double pm = (double)md.p;
     const double invp = md.invp;
 
-    // mod-p reduction of the INT32 accumulators after stage s when due
+    // mod-p reduction of the accumulators after stage s when due
     auto reduce = [&](int s) {
         if (!(ROTR & 2) && (s + 1) % md.chunk_stages == 0 && s + 1 < nst) {
 #pragma unroll
@@ -640,10 +687,26 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
                 for (int j = 0; j < CB; ++j)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        double x = (double)are[i][j][q];
-                        are[i][j][q] = (int)fma(-floor(x * invp), pm, x);
-                        x = (double)aim[i][j][q];
-                        aim[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                        if constexpr (ES == 1) {
+                            double x = (double)are[i][j][q];
+                            are[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                            x = (double)aim[i][j][q];
+                            aim[i][j][q] = (int)fma(-floor(x * invp), pm, x);
+                        } else {
+                            // exact integers below 2^24: into [0, p) in FP32 (the
+                            // quotient may be one off; the fma itself is exact)
+                            const float pf = (float)md.p, ipf = 1.0f / (float)md.p;
+                            float x = are[i][j][q];
+                            float r = fmaf(-floorf(x * ipf), pf, x);
+                            r += r < 0.0f ? pf : 0.0f;
+                            r -= r >= pf ? pf : 0.0f;
+                            are[i][j][q] = r;
+                            x = aim[i][j][q];
+                            r = fmaf(-floorf(x * ipf), pf, x);
+                            r += r < 0.0f ? pf : 0.0f;
+                            r -= r >= pf ? pf : 0.0f;
+                            aim[i][j][q] = r;
+                        }
                     }
         }
     };
@@ -665,23 +728,44 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
         }
     };
     auto mfmas = [&](const i4v* a, const i4v* b, const i4v* br) {
-        i4v ar[RB];
-        if (!(ROTR & 1)) {
+        if constexpr (ES == 2) {
+            i4v ar[RB];
 #pragma unroll
             for (int i = 0; i < RB; ++i)
-                ar[i] = i4v{rot_pairs(a[i][0]), rot_pairs(a[i][1]), rot_pairs(a[i][2]), rot_pairs(a[i][3])};
+                ar[i] = i4v{rot_bf16(a[i][0]), rot_bf16(a[i][1]), rot_bf16(a[i][2]), rot_bf16(a[i][3])};
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    are[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a[i]),
+                                                                        __builtin_bit_cast(bf8v, b[j]), are[i][j], 0,
+                                                                        0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    aim[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, ar[i]),
+                                                                        __builtin_bit_cast(bf8v, b[j]), aim[i][j], 0,
+                                                                        0, 0);
+        } else {
+            i4v ar[RB];
+            if (!(ROTR & 1)) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    ar[i] = i4v{rot_pairs(a[i][0]), rot_pairs(a[i][1]), rot_pairs(a[i][2]), rot_pairs(a[i][3])};
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    aim[i][j] = (ROTR & 1) ? __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], br[j], aim[i][j], 0, 0, 0)
+                                           : __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int j = 0; j < CB; ++j)
-                are[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], are[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < RB; ++i)
-#pragma unroll
-            for (int j = 0; j < CB; ++j)
-                aim[i][j] = (ROTR & 1) ? __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], br[j], aim[i][j], 0, 0, 0)
-                                       : __builtin_amdgcn_mfma_i32_16x16x64_i8(ar[i], b[j], aim[i][j], 0, 0, 0);
     };
 
     if constexpr ((ROTR & 4) == 0) {
@@ -696,7 +780,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
             asm volatile("s_barrier" ::: "memory");
             issue(min(s + NS - 1, nst - 1), (s + NS - 1) % NS);
 #pragma unroll
-            for (int ks = 0; ks < KS / 32; ++ks) {
+            for (int ks = 0; ks < Cfg::kKSteps; ++ks) {
                 i4v a[RB], b[CB], br[CB];
                 read_frags(s, ks, a, b, br);
                 mfmas(a, b, br);
@@ -708,7 +792,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
         // (across the stage boundary: after the barrier that publishes the
         // next stage) go out before this k-step's MFMAs, so the matrix pipe
         // keeps the MFMAs queued before the barrier busy while waves meet.
-        constexpr int NKS = KS / 32;
+        constexpr int NKS = Cfg::kKSteps;
         __builtin_amdgcn_s_waitcnt(vmcnt_only<LPW * (NS - 2)>());
         asm volatile("s_barrier" ::: "memory");
         issue(min(NS - 1, nst - 1), (NS - 1) % NS);
@@ -742,7 +826,21 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
-    store_residues<RB, CB, BM * BN>(md, tw.slot, wave, lane, are, aim);
+    if constexpr (ES == 1) {
+        store_residues<RB, CB, BM * BN>(md, tw.slot, wave, lane, are, aim);
+    } else {
+        i4v ire[RB][CB], iim[RB][CB];   // exact integers below 2^24
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    ire[i][j][q] = (int)are[i][j][q];
+                    iim[i][j][q] = (int)aim[i][j][q];
+                }
+        store_residues<RB, CB, BM * BN>(md, tw.slot, wave, lane, ire, iim);
+    }
 }
 
 #define HSGEN_IGEMM_CFGS(X)        \
@@ -784,19 +882,30 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_igemm2_crt(const IgemmProble
     Y(49, 128, 256, 32, 2, 4, 6, 0)      \
     Y(51, 256, 192, 64, 4, 2, 2, 0)
 
+// the same kernel on bf16 residues and BF16 MFMA: (id, BM, BN, KS, WGM, WGN, NS)
+#define HSGEN_BF16_CFGS(Z)              \
+    Z(61, 256, 192, 32, 4, 2, 2)         \
+    Z(62, 256, 128, 32, 4, 2, 3)         \
+    Z(63, 128, 256, 32, 2, 4, 3)
+
 IgemmLaunchCfg igemm_cfg(int id)
 {
 #define X(ID, BM, BN, BK, WGM, WGN) \
     if (id == ID) return IgemmLaunchCfg{BM, BN, BK, WGM * WGN * 64, IgemmCfg<BM, BN, BK, WGM, WGN>::kStageBytes * 2, 1, 0, \
-                                        WGM, WGN};
+                                        WGM, WGN, 1};
     HSGEN_IGEMM_CFGS(X)
 #undef X
 #define Y(ID, BM, BN, KS, WGM, WGN, NS, ROTR) \
     if (id == ID) return IgemmLaunchCfg{BM, BN, KS, WGM * WGN * 64, Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, ROTR>::kLdsBytes, \
-                                        (ROTR & 1) ? 2 : 0, (ROTR & 2) ? 65536 : 0, WGM, WGN};
+                                        (ROTR & 1) ? 2 : 0, (ROTR & 2) ? 65536 : 0, WGM, WGN, 1};
     HSGEN_IGEMM2_CFGS(Y)
 #undef Y
-    return IgemmLaunchCfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
+#define Z(ID, BM, BN, KS, WGM, WGN, NS) \
+    if (id == ID) return IgemmLaunchCfg{BM, BN, KS, WGM * WGN * 64, Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, 0, 2>::kLdsBytes, \
+                                        0, 0, WGM, WGN, 2};
+    HSGEN_BF16_CFGS(Z)
+#undef Z
+    return IgemmLaunchCfg{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
 
 hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work, int nwork,
@@ -829,6 +938,18 @@ hipError_t launch_igemm(int id, const IgemmProblem* probs, const TileWork* work,
     }
     HSGEN_IGEMM2_CFGS(Y)
 #undef Y
+#define Z(ID, BM, BN, KS, WGM, WGN, NS)                                                         \
+    if (id == ID) {                                                                              \
+        constexpr int lds = Igemm2Cfg<BM, BN, KS, WGM, WGN, NS, 0, 2>::kLdsBytes;                \
+        hipError_t e = hipFuncSetAttribute((const void*)k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, 0, 2>, \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);       \
+        if (e != hipSuccess) return e;                                                           \
+        hipLaunchKernelGGL((k_igemm2_crt<BM, BN, KS, WGM, WGN, NS, 0, 2>), dim3(nwork),           \
+                           dim3(WGM * WGN * 64), lds, st, probs, work, nwork, md);               \
+        return hipGetLastError();                                                                \
+    }
+    HSGEN_BF16_CFGS(Z)
+#undef Z
     return hipErrorInvalidValue;
 }
 

@@ -232,7 +232,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
     // folded) and rotated copies the kernel reads, 2 bytes per complex.  As many moduli as fit in 60% of
     // the free memory share one residue pass, which reads the FP64 operands
     // once for the whole group.
-    const int64_t ldo = 2 * kd;
+    const int64_t ldo = 2 * kd * ic.esize;   // bytes per column of a residue array
     if ((uint64_t)ldo * (uint64_t)(ic.bm + ic.bn) >= (1ull << 32))
         return fail(ctx, HSGEN_ERR_UNSUPPORTED, "INT8 contraction depth %lld too large for 32-bit tile offsets",
                     (long long)kd);
@@ -280,7 +280,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
         nprob = base.size();
         for (int c = 0; c < nchunk; ++c)
             for (IgemmProblem p : base) {
-                const int64_t off = 2 * (int64_t)c * kc;   // bytes
+                const int64_t off = 2 * ic.esize * (int64_t)c * kc;   // bytes
                 p.Lre += off;
                 if (p.Lrot) p.Lrot += off;
                 p.R += off;
@@ -391,6 +391,7 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
         rp.beta = pl.beta;
         rp.nmod = gn;
         rp.ldo = ldo;
+        rp.esize = ic.esize;
         for (int i = 0; i < gn; ++i) {
             rp.p[i] = kModuli[m0 + i];
             rp.invp[i] = 1.0 / kModuli[m0 + i];
@@ -432,7 +433,10 @@ hsgen_status oz_execute(hsgen_ctx* ctx, const Geometry& geo, const std::vector<O
                     IgemmModulus md;
                     md.p = kModuli[m];
                     md.first_chunk = c == 0;
-                    md.chunk_stages = 2048 / (ic.bk / 32);    // 2048 MFMA k-steps <= 2080 (|acc| < 2^31)
+                    // INT8: 2048 MFMA k-steps <= 2080 (|acc| < 2^31); BF16: 32 k-steps of
+                    // 32 x 127^2 each, exact in FP32 (|acc| < 2^24); a stage holds bk / 32
+                    // (INT8) or bk / 16 (BF16) k-steps
+                    md.chunk_stages = ic.esize == 1 ? 2048 / (ic.bk / 32) : 32 / (ic.bk / 16);
                     md.pad = 0;
                     md.invp = 1.0 / kModuli[m];
                     md.out = rout + (size_t)m * mod_stride;
@@ -786,11 +790,11 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
 {
     const int flags = o.flags;
     const bool do_s = !(flags & HSGEN_SKIP_S), do_h = !(flags & HSGEN_SKIP_H);
-    const int icfg = (o.tile > 0 && igemm_cfg(o.tile).bm > 0) ? o.tile : kIgemmCfg;
+    const int icfg = split_cfg(o.precision, o.tile);
     const IgemmLaunchCfg ic = igemm_cfg(icfg);
     int max_ng = 1;
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
-    const int64_t align = int8_col_align(icfg);
+    const int64_t align = split_col_align(o.precision, icfg);
     Geometry g0 = geometry(0, s->kdim, max_ng, align);
     g0.fold = s->fold;
     const int64_t tm = std::max<int64_t>({(int64_t)g0.t.bm, (int64_t)g0.t.bn, align});
@@ -843,7 +847,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
         for (size_t bi = 0; bi < ntr; ++bi)
             tiles += std::min(ntc, ((bi + 1) * ic.bm + ic.bn - 1) / ic.bn);
         need[i] = n_operands(g0) * sizeof(double2) * (size_t)g0.ldz * gp + sizeof(double2) * ng * ng * ((!hdev[i]) + (!sdev[i])) +
-                  (size_t)(do_s + do_h) * tiles * 2 * ic.bm * ic.bn * nmod_est + (size_t)gslots * narr * 2 * kd * gp;
+                  (size_t)(do_s + do_h) * tiles * 2 * ic.bm * ic.bn * nmod_est + (size_t)gslots * narr * 2 * ic.esize * kd * gp;
         total += need[i];
     }
     // consecutive k-points (largest first) into batches of at most `cap`

@@ -90,8 +90,7 @@ hsgen_status hsgen_autotune(hsgen_ctx* ctx, const hsgen_sys* sys, const hsgen_kp
     if (!ctx || !sys) return fail(ctx, HSGEN_ERR_ARG, "null argument");
     hsgen_status st = check_kpoint(ctx, kp);
     if (st) return st;
-    if (precision != HSGEN_PREC_FP64 && precision != HSGEN_PREC_OZAKI_INT8)
-        return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", precision);
+    if (!valid_precision(precision)) return fail(ctx, HSGEN_ERR_ARG, "unknown precision %d", precision);
     const int ng = kp->ng;
     if (ng <= 0) return fail(ctx, HSGEN_ERR_ARG, "autotuning needs ng > 0");
     HIPCHK(hipSetDevice(ctx->device));
@@ -100,8 +99,9 @@ hsgen_status hsgen_autotune(hsgen_ctx* ctx, const hsgen_sys* sys, const hsgen_kp
         for (int id = 1; id < 64; ++id)
             if (zgemm_cfg(id).bm > 0) cands.push_back(id);
     } else {
+        const int es = precision == HSGEN_PREC_OZAKI_BF16 ? 2 : 1;   // the precision's MFMA type
         for (int id = 1; id < 64; ++id)
-            if (igemm_cfg(id).bm > 0) cands.push_back(id);
+            if (igemm_cfg(id).bm > 0 && igemm_cfg(id).esize == es) cands.push_back(id);
     }
     // device outputs owned by the call
     st = ensure(ctx, ctx->Htmp, sizeof(double2) * (size_t)ng * ng, false);

@@ -42,10 +42,15 @@ typedef enum {
  * final recombination are always FP64). */
 typedef enum {
     HSGEN_PREC_FP64 = 0,      /* FP64 MFMA (v_mfma_f64_16x16x4) */
-    HSGEN_PREC_OZAKI_INT8 = 1 /* split precision: INT8 slices on INT8 MFMA,
+    HSGEN_PREC_OZAKI_INT8 = 1, /* split precision: INT8 slices on INT8 MFMA,
                                  exact INT32 accumulation, FP64 recombination;
                                  HSGEN_ERR_UNSUPPORTED if 2 (2K) bytes x 448 columns
                                  of an operand would exceed 4 GiB (K > 2.3e6) */
+    HSGEN_PREC_OZAKI_BF16 = 2 /* the same slices (residues) held as bf16 on
+                                 BF16 MFMA, exact FP32 accumulation: the same H
+                                 and S bit for bit as HSGEN_PREC_OZAKI_INT8, at
+                                 half its nominal MFMA rate and twice its slice
+                                 memory (K > 1.1e6 unsupported) */
 } hsgen_precision;
 
 enum {
@@ -90,7 +95,7 @@ typedef struct {
 
 typedef struct {
     int32_t precision;         /* hsgen_precision */
-    int32_t slices;            /* split precision: INT8 residue slices per operand, i.e. the
+    int32_t slices;            /* split precision: residue slices per operand, i.e. the
                                   number of moduli (0 = chosen for target_rel_err from an error
                                   model and an FP64 sample of columns of H and S, then checked
                                   against the model with the exact norms, with a retry) */
@@ -114,7 +119,7 @@ typedef struct {
     int32_t tile_used;         /* contraction tile config id actually used */
     double err_est_h;          /* split precision: estimated relative Frobenius error of H */
     double err_est_s;          /* ... and of S (0 in FP64) */
-    double ms_igemm;           /* split precision: GPU time of the INT8 GEMM launches alone
+    double ms_igemm;           /* split precision: GPU time of the residue (INT8 or BF16) GEMM launches alone
                                   (last modulus set when an automatic retry ran) */
     double err_smp_h;          /* split precision: measured relative Frobenius error of H on
                                   64 evenly spaced columns (their lower parts) recomputed in FP64 */

@@ -266,6 +266,12 @@ def test_split_precision_deep_contraction(ctx):
         assert np.array_equal(Hb, H) and np.array_equal(Sb, S), k.ng
         for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
             assert rb[key] == r[key], (k.ng, key)
+    # BF16 MFMA over a depth of 1057 stages (its FP32 accumulators reduced mod p
+    # every 16): the INT8 bits
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_BF16
+    Hb, Sb = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, target_rel_err=1e-12)
+    Hi, Si = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    assert np.array_equal(Hb, Hi) and np.array_equal(Sb, Si)
 
 
 def test_argument_errors(ctx, big_list):

@@ -232,6 +232,33 @@ def test_split_precision_equals_rounding_model(ctx, monkeypatch, fold):
     assert rel_fro_lower(H, Hm) <= 1e-11
 
 
+@pytest.mark.parametrize("tile", [0, 62, 63])
+def test_bf16_slices_equal_int8_slices_bitwise(ctx, tile):
+    """The BF16 variant multiplies the same residues (held as bf16) on BF16
+    MFMA with FP32 accumulation, exact below 2^24 and reduced mod p every 32
+    MFMAs: its H and S are the INT8 path's bit for bit, for an automatic
+    modulus count and a fixed one, single and batched, on every BF16 tile."""
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8
+    s, kp = tiny_system(natoms=4, lmax=(6, 8), ng_target=300, seed=3)
+    h = ctx.upload(s)
+    for kw in ({"target_rel_err": 1e-12}, {"slices": 8}):
+        Hi, Si, ri = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, report=True, **kw)
+        Hb, Sb, rb = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, tile=tile, report=True, **kw)
+        print(f"tile {tile} {kw}: tiles used {ri['tile_used']} / {rb['tile_used']}, moduli {rb['slices_used']}",
+              flush=True)
+        assert rb["tile_used"] == (tile or 61) and ri["tile_used"] == 51
+        assert np.array_equal(Hb, Hi) and np.array_equal(Sb, Si)
+        for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
+            assert rb[key] == ri[key], key
+    Hr, Sr = po.reference_hs(s, kp)
+    assert rel_fro_lower(Hb, Hr) > 1e-11       # the 8-moduli result: a real rounding error
+    kps = [kp, tiny_system(natoms=4, lmax=(6, 8), ng_target=150, seed=3, kpt=(0.3, 0.1, 0.0))[1]]
+    res, _ = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_BF16, target_rel_err=1e-12, tile=tile)
+    for k, (Hb, Sb) in zip(kps, res):
+        Hi, Si = ctx.create_HS2(h, k, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+        assert np.array_equal(Hb, Hi) and np.array_equal(Sb, Si)
+
+
 def test_split_precision_tile_configs_bitwise_equal(ctx):
     """Every INT8 kernel configuration computes the same exact integer
     products and the same recombination, so the results agree bit for bit."""
@@ -398,8 +425,14 @@ def test_autotune_picks_and_applies_a_tile(ctx):
     bi, msi = ctx.autotune(h, kp, precision=HSGEN_PREC_OZAKI_INT8)
     print(f"autotune INT8: tile {bi} ({msi:.3f} ms)", flush=True)
     H, S, rep = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
-    assert rep["tile_used"] == bi
+    assert rep["tile_used"] == bi and bi < 61
     assert rel_fro_lower(H, Hr) <= 1e-12 and rel_fro_lower(S, Sr) <= 1e-12
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_BF16
+    bb, msb = ctx.autotune(h, kp, precision=HSGEN_PREC_OZAKI_BF16)
+    print(f"autotune BF16: tile {bb} ({msb:.3f} ms)", flush=True)
+    Hb, Sb, rep = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, target_rel_err=1e-12, report=True)
+    assert rep["tile_used"] == bb and bb in (61, 62, 63)
+    assert np.array_equal(Hb, H) and np.array_equal(Sb, S)
 
 
 def test_int8_mfma_ceiling_diagnostic(ctx):
