@@ -121,6 +121,24 @@ def build_io() -> Path | None:
     return IO_LIB
 
 
+DRIVER = LIBDIR / "hsgen_driver"
+
+
+def build_driver() -> Path | None:
+    """hsgen_driver: the reference's h_and_s executable over the C ABI (plain
+    C++11 against include/, linked to libhsgen_io.so and libhsgen.so).
+    Skipped when the dump I/O library is not built (no HDF5)."""
+    if not IO_LIB.exists():
+        print("libhsgen_io.so not built: hsgen_driver not built", file=sys.stderr)
+        return None
+    tmp = DRIVER.with_suffix(".tmp")
+    _run([shutil.which("g++") or "g++", "-O2", "-std=c++11", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"),
+          "-o", str(tmp), str(CSRC / "hsgen_driver.cpp"), "-L", str(LIBDIR), "-lhsgen_io", "-lhsgen",
+          "-Wl,--enable-new-dtags", "-Wl,-rpath,$ORIGIN:/opt/rocm/lib"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def _drain(procs) -> None:
     while procs:
         cmd, p = procs.pop(0)
@@ -151,6 +169,9 @@ def main(argv=None) -> int:
         io = build_io()
         if io:
             print("built", io)
+            drv = build_driver()
+            if drv:
+                print("built", drv)
     if not args.lib_only:
         print("built", build_oracle())
     return 0

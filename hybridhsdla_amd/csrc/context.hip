@@ -336,6 +336,20 @@ const char* hsgen_last_error(const hsgen_ctx* ctx)
     return ctx->err.c_str();
 }
 
+namespace {
+
+// The context's stream and timing events on its device.
+hsgen_status init_stream(hsgen_ctx* ctx)
+{
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
+    for (auto& ev : ctx->oz_ev) HIPCHK(hipEventCreate(&ev));
+    return HSGEN_OK;
+}
+
+}  // namespace
+
 hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
 {
     hsgen_ctx* ctx = nullptr;
@@ -354,11 +368,9 @@ hsgen_status hsgen_init(hsgen_ctx** out, int32_t device)
     ctx = new hsgen_ctx();
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
-    HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    for (auto& ev : ctx->ev) HIPCHK(hipEventCreate(&ev));
-    for (auto& ev : ctx->oz_ev) HIPCHK(hipEventCreate(&ev));
-    const hsgen_status st = noalloc_setup(ctx);
+    // a failed set-up releases what was created: the caller gets no context
+    hsgen_status st = init_stream(ctx);
+    if (st == HSGEN_OK) st = noalloc_setup(ctx);
     if (st != HSGEN_OK) {
         hsgen_finalize(ctx);
         return st;

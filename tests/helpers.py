@@ -42,3 +42,16 @@ def stratified_sample(ng, block=128, seed=1):
     """One index drawn from every block of `block` consecutive indices."""
     rng = np.random.default_rng(seed)
     return np.array([b0 + int(rng.integers(0, min(block, ng - b0))) for b0 in range(0, ng, block)], dtype=np.int64)
+
+
+def driver_or_fail():
+    """Path of the built C++ driver program (hsgen_driver).  Where HDF5 is
+    installed build() builds it, so a missing program is a failure; only a
+    machine without HDF5 skips, and says so."""
+    import pytest
+    from hybridhsdla_amd import build
+    if build.DRIVER.exists():
+        return build.DRIVER
+    if build.hdf5_available():
+        pytest.fail(f"HDF5 is installed under {build.HDF5_ROOT} but {build.DRIVER.name} is not built: run build()")
+    pytest.skip(f"no HDF5 under {build.HDF5_ROOT}: the driver program is not built")

@@ -1,8 +1,12 @@
 """The C ABI libraries load on a machine without a GPU and export every
-function their headers declare (no compute calls here), and the ctypes
-binding declares each of them."""
+function their headers declare (no compute calls here), the ctypes binding
+declares each of them, the headers compile as C99 and as C++11 (the
+reference's language), and the C++ driver program starts and refuses bad
+arguments before touching a GPU."""
 import ctypes
 import re
+import shutil
+import subprocess
 from pathlib import Path
 
 import pytest
@@ -47,3 +51,31 @@ def test_status_strings_without_gpu():
     assert lib.hsgen_status_string(0) == b"ok"
     assert lib.hsgen_status_string(-6) == b"accuracy target unreachable"
     assert lib.hsgen_last_error(None) == b"null context"
+
+
+@pytest.mark.parametrize("lang,std,compiler", [("c", "c99", "gcc"), ("c++", "c++11", "g++")])
+def test_headers_compile_standalone(tmp_path, lang, std, compiler):
+    cc = shutil.which(compiler)
+    if cc is None:
+        pytest.skip(f"{compiler} not installed")
+    src = tmp_path / ("use." + ("c" if lang == "c" else "cpp"))
+    src.write_text(
+        '#include "hsgen.h"\n#include "hsgen_io.h"\n'
+        "int probe(hsgen_ctx* c, const hsgen_sys* s, const hsgen_kpoint* k, double* h) {\n"
+        "    hsgen_opts o = {HSGEN_PREC_OZAKI_BF16, 0, 1e-12, HSGEN_FILL_UPPER, 0};\n"
+        "    hsgen_report r;\n"
+        "    return (int)hsgen_create_hs(c, s, k, &o, h, h, k->ng, &r);\n"
+        "}\n")
+    res = subprocess.run([cc, f"-std={std}", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                          "-I", str(ROOT / "include"), str(src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+def test_driver_refuses_bad_arguments_without_a_gpu():
+    from tests.helpers import driver_or_fail
+    drv = driver_or_fail()
+    for args in ([], ["dump.h5", "--precision", "fp32"], ["dump.h5", "--bogus"], ["dump.h5", "0"]):
+        res = subprocess.run([str(drv), *args], capture_output=True, text=True, timeout=60)
+        assert res.returncode == 2 and "usage: hsgen_driver" in res.stderr, (args, res.returncode, res.stderr)
+    res = subprocess.run([str(drv), str(ROOT / "no_such_dump.h5")], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 1 and "hsgen_driver:" in res.stderr

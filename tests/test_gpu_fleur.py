@@ -1,12 +1,16 @@
 """End to end on the GPU against Fleur's own H and S (the reference's only
 golden data, tests/golden/two_atoms/fleur_dump.h5): the dump is read and the
 radial functions and final T computed on the host, then A/B generation and
-the contraction run through the C ABI on the GPU.
+the contraction run through the C ABI on the GPU.  The same comparison runs
+in the C++ driver program (the reference's h_and_s executable over the C ABI).
 """
+import re
+import subprocess
+
 import numpy as np
 import pytest
 
-from tests.helpers import open_dump_or_fail, rel_fro_lower
+from tests.helpers import driver_or_fail, open_dump_or_fail, rel_fro_lower
 
 pytestmark = pytest.mark.gpu
 
@@ -80,3 +84,22 @@ def test_small_workload_through_the_dump_layout(ctx, tmp_path):
     with dump_mod.HSStore(tmp_path / "hs.h5") as st:
         st.add(0, H, S)
     assert (tmp_path / "hs.h5").stat().st_size > 2 * 16 * kp2.ng * (kp2.ng + 1) // 2
+
+
+@pytest.mark.parametrize("args,tol", [([], 2e-14), (["--batched"], 2e-14),
+                                      (["--precision", "int8", "--target", "1e-12"], 1e-12),
+                                      (["--precision", "bf16", "--target", "1e-12"], 1e-12)])
+def test_cpp_driver_reproduces_fleur(tmp_path, args, tol):
+    """hsgen_driver reads the dump, builds H and S on the GPU, writes them in
+    the reference's output layout and compares them with Fleur's own."""
+    drv = driver_or_fail()
+    out = tmp_path / "flapw_out.h5"
+    cmd = [str(drv), FIXTURE, "--out", str(out), "--check", str(tol), *args]
+    print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr, flush=True)
+    assert res.returncode == 0, res.stderr
+    m = re.search(r"rel H ([0-9.e+-]+) S ([0-9.e+-]+)", res.stdout)
+    assert m, res.stdout
+    assert float(m.group(1)) <= tol and float(m.group(2)) <= tol
+    assert out.exists() and out.stat().st_size > 2 * 16 * 235 * 235
