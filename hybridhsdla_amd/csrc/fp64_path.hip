@@ -406,7 +406,9 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
         if (!cur.empty()) batches.push_back(cur);
     }
     if (ctx->bgraphs.size() < batches.size()) ctx->bgraphs.resize(batches.size());
-    const int pgrid = ctx->n_cu > 0 ? ctx->n_cu : 256;   // one 8-wave workgroup per CU (LDS-limited)
+    // persistent grids: every workgroup slot of every CU (one for the default
+    // tile, whose 128 KiB of LDS fill a CU)
+    const int ncu = ctx->n_cu > 0 ? ctx->n_cu : 256;
     double ms_all = 0.0;
     for (size_t b = 0; b < batches.size(); ++b) {
         const std::vector<int>& ks = batches[b];
@@ -511,6 +513,8 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
                                       hipMemcpyHostToDevice, ctx->stream));
             // the host vectors must outlive the copies
             HIPCHK(hipStreamSynchronize(ctx->stream));
+            const int grid_t = tc == geo.cfg ? ncu * zgemm_blocks_per_cu(tc) : 0;
+            const int grid_c = ncu * zgemm_blocks_per_cu(geo.cfg);
             // capture the launch sequence
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             hipError_t ce = hipSuccess;
@@ -527,12 +531,12 @@ hsgen_status batched_fp64(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpoi
             const GemmProblem* dp = (const GemmProblem*)bg.probs.p;
             const TileWork* dw = (const TileWork*)bg.work.p;
             if (ce == hipSuccess && do_h) {
-                ce = launch_zgemm(tc, dp, dw, (int)tw_t.size(), ctx->stream, tc == geo.cfg ? pgrid : 0);
+                ce = launch_zgemm(tc, dp, dw, (int)tw_t.size(), ctx->stream, grid_t);
                 if (ce == hipSuccess)
                     ce = launch_clear_rows(W, geo.ldz, 2 * (int64_t)s->kdim, geo.ldz, (int)ncols, ctx->stream);
             }
             if (ce == hipSuccess)
-                ce = launch_zgemm(geo.cfg, dp, dw + tw_t.size(), (int)tw_c.size(), ctx->stream, pgrid);
+                ce = launch_zgemm(geo.cfg, dp, dw + tw_t.size(), (int)tw_c.size(), ctx->stream, grid_c);
             for (size_t q = 0; q < ks.size() && ce == hipSuccess && (flags & HSGEN_FILL_UPPER); ++q) {
                 if (do_h) ce = launch_fill_upper(dH[q], ldH[q], kps[ks[q]].ng, ctx->stream);
                 if (ce == hipSuccess && do_s) ce = launch_fill_upper(dS[q], ldS[q], kps[ks[q]].ng, ctx->stream);

@@ -109,6 +109,7 @@ ZgemmLaunchCfg zgemm_cfg(int id);
 bool zgemm_offs32(int id);   // operand panels addressed with 32-bit offsets: BM or BN columns x ld < 4 GiB
 hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,
                         hipStream_t st, int max_grid = 0);
+int zgemm_blocks_per_cu(int id);   // resident workgroups per CU (persistent grids)
 hipError_t launch_clear_rows(double2* X, int64_t ld, int64_t r0, int64_t r1, int ncols, hipStream_t st);
 hipError_t launch_fill_upper(double2* C, int64_t ld, int n, hipStream_t st);
 

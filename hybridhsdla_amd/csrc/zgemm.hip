@@ -177,7 +177,7 @@ __device__ __forceinline__ void zgemm_tile(const GemmProblem* __restrict__ probs
 // next k-step are read before the MFMAs of the current one, across the stage
 // barrier too.  LDS stage layout: column c (L columns first, then R) holds
 // its BK complex values as BK chunks of 16 bytes; logical chunk q sits at
-// position q ^ (c % 16), which makes the 16-lane fragment reads (16
+// position q ^ zswz<BK>(c), which makes the 16-lane fragment reads (16
 // consecutive columns, one chunk) hit 16 distinct bank groups.  Same MFMA
 // sequence per accumulator as zgemm_tile, so the same bits.
 // ---------------------------------------------------------------------------
@@ -194,11 +194,20 @@ struct Zgemm2Cfg {
     static constexpr int kWaveLoads = (BM + BN) * BK / 64;       // 1 KiB wave loads per stage
     static constexpr int kLoadsPerWave = kWaveLoads / kWaves;
     static constexpr int kLdsBytes = 2 * kStageBytes;
-    static_assert(BK == 16, "the swizzle spreads 16 consecutive columns over 16 chunks");
+    static_assert(BK == 16 || BK == 8, "the swizzle spreads 16 consecutive columns over 16 bank groups");
     static_assert(kWM % 16 == 0 && kWN % 16 == 0, "wave tile must be a multiple of 16");
     static_assert(kWaveLoads % kWaves == 0, "stage must split evenly over waves");
     static_assert(BM % (64 / BK) == 0, "a wave load must stay in one operand");
 };
+
+// Chunk swizzle of column c: with 16 chunks per column (256 bytes) c % 16
+// picks the chunk; with 8 (128 bytes, two columns per 256-byte bank row) the
+// column's parity picks the half row and (c / 2) % 8 the chunk.
+template <int BK>
+__device__ __forceinline__ int zswz(int c)
+{
+    return BK == 16 ? (c & 15) : ((c >> 1) & 7);
+}
 
 // s_waitcnt immediates: vmcnt = 0 and nothing else; vmcnt = 0 and lgkmcnt = 0
 // (vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
@@ -238,7 +247,7 @@ __device__ __forceinline__ void zgemm2_tile(const GemmProblem* __restrict__ prob
         const int c0 = g * (64 / BK);
         const int slot = g * 64 + lane;
         const int c = slot / BK;
-        const int q = (slot % BK) ^ (c % 16);
+        const int q = (slot % BK) ^ zswz<BK>(c);
         if (c0 < BM) {
             base[r] = (const char*)(pb.L + (int64_t)I0 * pb.ldl);
             off[r] = (uint32_t)(((int64_t)c * pb.ldl + q) * 16);
@@ -281,10 +290,10 @@ __device__ __forceinline__ void zgemm2_tile(const GemmProblem* __restrict__ prob
         const char* lb = lds + buf * SB;
 #pragma unroll
         for (int i = 0; i < RB; ++i)
-            a[i] = *(const v2d*)&lb[colA[i] * CBY + 16 * ((q0 + 4 * ks) ^ (colA[i] % 16))];
+            a[i] = *(const v2d*)&lb[colA[i] * CBY + 16 * ((q0 + 4 * ks) ^ zswz<BK>(colA[i]))];
 #pragma unroll
         for (int j = 0; j < CB; ++j)
-            b[j] = *(const v2d*)&lb[colB[j] * CBY + 16 * ((q0 + 4 * ks) ^ (colB[j] % 16))];
+            b[j] = *(const v2d*)&lb[colB[j] * CBY + 16 * ((q0 + 4 * ks) ^ zswz<BK>(colB[j]))];
     };
     auto mfmas = [&](const v2d* a, const v2d* b) {
 #pragma unroll
@@ -412,7 +421,9 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_zgemm_lh(const GemmProblem* 
     Y(9, 128, 128, 16, 4, 2)        \
     Y(10, 128, 128, 16, 2, 4)       \
     Y(11, 128, 128, 16, 4, 4)       \
-    Y(12, 64, 64, 16, 2, 2)
+    Y(12, 64, 64, 16, 2, 2)         \
+    Y(13, 64, 128, 8, 2, 4)         \
+    Y(14, 128, 64, 8, 4, 2)
 
 // the pipelined kernels address a tile's operand panel with 32-bit offsets
 bool zgemm_offs32(int id)
@@ -432,6 +443,31 @@ ZgemmLaunchCfg zgemm_cfg(int id)
     HSGEN_ZGEMM2_CFGS(X)
 #undef X
     return ZgemmLaunchCfg{0, 0, 0, 0};
+}
+
+// Workgroups of configuration id that one CU holds at once (registers, LDS
+// and waves permitting): a persistent launch sizes its grid by it.
+int zgemm_blocks_per_cu(int id)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define X(ID, BM, BN, BK, WGM, WGN)                                                                   \
+    if (id == ID)                                                                                     \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_zgemm_lh<BM, BN, BK, WGM, WGN>, \
+                                                         WGM * WGN * 64, 0);
+    HSGEN_ZGEMM_CFGS(X)
+#undef X
+#define Y(ID, BM, BN, BK, WGM, WGN)                                                                     \
+    if (id == ID)                                                                                       \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_zgemm2_lh<BM, BN, BK, WGM, WGN>, \
+                                                         WGM * WGN * 64, Zgemm2Cfg<BM, BN, BK, WGM, WGN>::kLdsBytes);
+    HSGEN_ZGEMM2_CFGS(Y)
+#undef Y
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
 }
 
 hipError_t launch_zgemm(int id, const GemmProblem* probs, const TileWork* work, int nwork,

@@ -106,13 +106,13 @@ def test_tile_configs_agree(ctx):
     s, kp = tiny_system(natoms=3, lmax=(3, 2, 4), ng_target=200)
     h = ctx.upload(s)
     H1, S1 = ctx.create_HS2(h, kp, tile=1)
-    for t in (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12):
+    for t in (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14):
         H2, S2 = ctx.create_HS2(h, kp, tile=t)
         assert rel_fro_lower(H2, H1) <= 1e-14
         assert rel_fro_lower(S2, S1) <= 1e-14
 
 
-@pytest.mark.parametrize("pipelined,staged", [(9, 8), (10, 2), (11, 8), (12, 3)])
+@pytest.mark.parametrize("pipelined,staged", [(9, 8), (10, 2), (11, 8), (12, 3), (13, 4), (14, 4)])
 def test_pipelined_fp64_kernel_is_bitwise_the_staged_one(ctx, pipelined, staged):
     """The LDS-DMA pipelined GEMM (operands straight into a swizzled LDS
     ring, fragments read ahead) runs each accumulator's MFMAs in the same
@@ -287,20 +287,21 @@ def _kset(s, nk=5, seed=3):
     return kps
 
 
-@pytest.mark.parametrize("budget", [None, "2000000"])
-def test_batched_equals_single_k(ctx, monkeypatch, budget):
+@pytest.mark.parametrize("budget,tile", [(None, 0), ("2000000", 0), (None, 14), (None, 12)])
+def test_batched_equals_single_k(ctx, monkeypatch, budget, tile):
     """All k-points in one grouped launch (or, with a small memory cap,
     several batches) give the single-k results bit for bit (same tiles, same
-    summation order) and the oracle's to 1e-13."""
+    summation order) and the oracle's to 1e-13; also with tiles of which a
+    CU holds several workgroups (a persistent grid of every slot)."""
     if budget:
         monkeypatch.setenv("HSGEN_BATCH_BYTES", budget)
     s, _ = tiny_system(natoms=3, lmax=(3, 4, 2), a_cell=7.0)
     kps = _kset(s)
-    print("batched ng:", [kp.ng for kp in kps], "budget", budget, flush=True)
+    print("batched ng:", [kp.ng for kp in kps], "budget", budget, "tile", tile, flush=True)
     h = ctx.upload(s)
-    res, reps = ctx.create_HS2_batched(h, kps)
+    res, reps = ctx.create_HS2_batched(h, kps, tile=tile)
     for kp, (H, S), r in zip(kps, res, reps):
-        H1, S1 = ctx.create_HS2(h, kp)
+        H1, S1 = ctx.create_HS2(h, kp, tile=tile)
         assert np.array_equal(H, H1) and np.array_equal(S, S1)
         assert r["ng"] == kp.ng
     Hr, Sr = po.reference_hs(s, kps[0])
