@@ -710,8 +710,12 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32
     if (o.precision == HSGEN_PREC_FP64) {
         HIPCHK(hipSetDevice(ctx->device));
         int max_ng = 1;
-        for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
-        o.tile = pick_tile(ctx, o, max_ng, sys->kdim);
+        int64_t tiles = 0;
+        for (int i = 0; i < nk; ++i) {
+            max_ng = std::max(max_ng, kps[i].ng);
+            tiles += lower_tiles128(kps[i].ng);
+        }
+        o.tile = pick_tile(ctx, o, max_ng, sys->kdim, std::max<int64_t>(tiles, 1));
         return batched_fp64(ctx, const_cast<hsgen_sys*>(sys), nk, kps, o, H, S, ld, rep);
     }
     if (o.slices < 0 || o.slices > kMaxModuli)

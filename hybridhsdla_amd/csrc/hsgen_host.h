@@ -35,6 +35,12 @@
 namespace hsgen {
 
 constexpr int kDefaultTile = 11;
+// FP64 tile for problems too small to fill the GPU with 128 x 128 tiles:
+// fewer than kSmallWaves tiles of 128 x 128 per CU (S and H lower tiles of
+// all k-points of the call) take the 64 x 64 tile, whose workgroups are a
+// quarter of the work and two of which a CU holds at once
+constexpr int kSmallTile = 12;
+constexpr int kSmallWaves = 6;
 constexpr int kFallbackTile = 8;   // register-staged, 64-bit addressing
 constexpr int kIgemmCfg = 51;      // default INT8 kernel configuration (256 x 192 tiles)
 constexpr int kBf16Cfg = 61;       // default BF16 kernel configuration (256 x 192 tiles)
@@ -234,6 +240,10 @@ hsgen_status noalloc_setup(hsgen_ctx* ctx);
 std::vector<int> tune_key(int precision, int ng, int kdim);
 // Tile configuration for a call: the caller's, else the autotuned one for
 // this size, else 0 (the built-in default).
-int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim);
+// The tile of a call: opts.tile if set, else the autotuned one for the size,
+// else (FP64) the default by size; tiles128 = the call's lower S and H tiles
+// of 128 x 128 (0: those of one k-point of ng)
+int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, int64_t tiles128 = 0);
+int64_t lower_tiles128(int ng);
 
 }  // namespace hsgen

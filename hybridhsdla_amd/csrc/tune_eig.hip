@@ -71,11 +71,20 @@ std::vector<int> tune_key(int precision, int ng, int kdim)
     return {precision, b(ng), b(kdim)};
 }
 
-int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim)
+int64_t lower_tiles128(int ng)
+{
+    const int64_t n = (std::max(ng, 0) + 127) / 128;
+    return n * (n + 1);   // S and H
+}
+
+int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, int64_t tiles128)
 {
     if (o.tile > 0) return o.tile;
     auto it = ctx->tuned.find(tune_key(o.precision, ng, kdim));
-    return it == ctx->tuned.end() ? 0 : it->second;
+    if (it != ctx->tuned.end()) return it->second;
+    if (o.precision != HSGEN_PREC_FP64) return 0;
+    if (tiles128 <= 0) tiles128 = lower_tiles128(ng);
+    return tiles128 < (int64_t)kSmallWaves * std::max(ctx->n_cu, 1) ? kSmallTile : 0;
 }
 
 }  // namespace hsgen
