@@ -655,7 +655,7 @@ hsgen_status hsgen_create_hs(hsgen_ctx* ctx, const hsgen_sys* sc, const hsgen_kp
     if (rep) std::memset(rep, 0, sizeof *rep);
     if (ng == 0) return HSGEN_OK;
     HIPCHK(hipSetDevice(ctx->device));
-    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s->kdim, ng,
+    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : fp64_tile_by_size(ctx, ng), s->kdim, ng,
                             o.precision == HSGEN_PREC_FP64 ? 1 : split_col_align(o.precision, o.tile));
     geo.fold = s->fold;
     st = upload_g(ctx, kp);
@@ -710,12 +710,8 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32
     if (o.precision == HSGEN_PREC_FP64) {
         HIPCHK(hipSetDevice(ctx->device));
         int max_ng = 1;
-        int64_t tiles = 0;
-        for (int i = 0; i < nk; ++i) {
-            max_ng = std::max(max_ng, kps[i].ng);
-            tiles += lower_tiles128(kps[i].ng);
-        }
-        o.tile = pick_tile(ctx, o, max_ng, sys->kdim, std::max<int64_t>(tiles, 1));
+        for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
+        o.tile = pick_tile(ctx, o, max_ng, sys->kdim, kps, nk);
         return batched_fp64(ctx, const_cast<hsgen_sys*>(sys), nk, kps, o, H, S, ld, rep);
     }
     if (o.slices < 0 || o.slices > kMaxModuli)
@@ -723,7 +719,7 @@ hsgen_status hsgen_create_hs_batched(hsgen_ctx* ctx, const hsgen_sys* sys, int32
     HIPCHK(hipSetDevice(ctx->device));
     int max_ng = 1;
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
-    o.tile = pick_tile(ctx, o, max_ng, sys->kdim);
+    o.tile = pick_tile(ctx, o, max_ng, sys->kdim, kps, nk);
     return batched_split(ctx, const_cast<hsgen_sys*>(sys), nk, kps, o, H, S, ld, rep);
 }
 
@@ -785,7 +781,7 @@ hsgen_status hsgen_contract_hs(hsgen_ctx* ctx, int32_t L, int32_t a, int32_t G, 
     s.fold = fold;
     if (G == 0) return HSGEN_OK;
     o.tile = pick_tile(ctx, o, G, s.kdim);
-    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : 0, s.kdim, G,
+    Geometry geo = geometry(o.precision == HSGEN_PREC_FP64 ? o.tile : fp64_tile_by_size(ctx, G), s.kdim, G,
                             o.precision == HSGEN_PREC_FP64 ? 1 : split_col_align(o.precision, o.tile));
     geo.fold = s.fold;
     hsgen_status st = upload_M(ctx, &s, t_size, t_aa, t_ab, t_bb, s.fold ? s.h_sqrtn.data() : nullptr, geo.t.bk,

@@ -41,6 +41,10 @@ constexpr int kDefaultTile = 11;
 // quarter of the work and two of which a CU holds at once
 constexpr int kSmallTile = 12;
 constexpr int kSmallWaves = 6;
+// INT8 configuration for calls with few S and H lower tiles of 256 x 192 (all
+// k-points): 128 x 128 below kInt8FewTiles, 128 x 256 below kInt8SomeTiles
+constexpr int kInt8SmallCfg = 14, kInt8MidCfg = 15;
+constexpr int kInt8FewTiles = 64, kInt8SomeTiles = 160;
 constexpr int kFallbackTile = 8;   // register-staged, 64-bit addressing
 constexpr int kIgemmCfg = 51;      // default INT8 kernel configuration (256 x 192 tiles)
 constexpr int kBf16Cfg = 61;       // default BF16 kernel configuration (256 x 192 tiles)
@@ -241,9 +245,12 @@ std::vector<int> tune_key(int precision, int ng, int kdim);
 // Tile configuration for a call: the caller's, else the autotuned one for
 // this size, else 0 (the built-in default).
 // The tile of a call: opts.tile if set, else the autotuned one for the size,
-// else (FP64) the default by size; tiles128 = the call's lower S and H tiles
-// of 128 x 128 (0: those of one k-point of ng)
-int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, int64_t tiles128 = 0);
-int64_t lower_tiles128(int ng);
+// else the default by size over the call's k-points (kps[0 .. nk-1], or one
+// k-point of ng when kps is null): see fp64_tile_by_size and int8_cfg_by_size.
+int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, const hsgen_kpoint* kps = nullptr,
+              int nk = 0);
+// FP64 tile by size: kSmallTile when the S and H lower tiles of 128 x 128 of
+// all the k-points number fewer than kSmallWaves per CU, else 0 (the default)
+int fp64_tile_by_size(const hsgen_ctx* ctx, int ng, const hsgen_kpoint* kps = nullptr, int nk = 0);
 
 }  // namespace hsgen

@@ -795,7 +795,7 @@ hsgen_status batched_split(hsgen_ctx* ctx, hsgen_sys* s, int nk, const hsgen_kpo
     int max_ng = 1;
     for (int i = 0; i < nk; ++i) max_ng = std::max(max_ng, kps[i].ng);
     const int64_t align = split_col_align(o.precision, icfg);
-    Geometry g0 = geometry(0, s->kdim, max_ng, align);
+    Geometry g0 = geometry(fp64_tile_by_size(ctx, max_ng, kps, nk), s->kdim, max_ng, align);
     g0.fold = s->fold;
     const int64_t tm = std::max<int64_t>({(int64_t)g0.t.bm, (int64_t)g0.t.bn, align});
     const int64_t kd = roundup(g0.kpad, ic.bk);

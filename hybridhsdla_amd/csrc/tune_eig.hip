@@ -71,20 +71,45 @@ std::vector<int> tune_key(int precision, int ng, int kdim)
     return {precision, b(ng), b(kdim)};
 }
 
-int64_t lower_tiles128(int ng)
+namespace {
+
+// S and H lower tiles of bm x bn of one k-point of ng: the tiles whose first
+// row index is at least their first column index
+int64_t lower_tiles(int ng, int bm, int bn)
 {
-    const int64_t n = (std::max(ng, 0) + 127) / 128;
-    return n * (n + 1);   // S and H
+    const int64_t ntr = (std::max(ng, 0) + bm - 1) / bm, ntc = (std::max(ng, 0) + bn - 1) / bn;
+    int64_t n = 0;
+    for (int64_t bi = 0; bi < ntr; ++bi) n += std::min<int64_t>(ntc, ((bi + 1) * bm + bn - 1) / bn);
+    return 2 * n;
 }
 
-int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, int64_t tiles128)
+int64_t call_tiles(int ng, const hsgen_kpoint* kps, int nk, int bm, int bn)
+{
+    if (!kps) return lower_tiles(ng, bm, bn);
+    int64_t n = 0;
+    for (int i = 0; i < nk; ++i) n += lower_tiles(kps[i].ng, bm, bn);
+    return n;
+}
+
+}  // namespace
+
+int fp64_tile_by_size(const hsgen_ctx* ctx, int ng, const hsgen_kpoint* kps, int nk)
+{
+    const int64_t t = call_tiles(ng, kps, nk, 128, 128);
+    return t < (int64_t)kSmallWaves * std::max(ctx->n_cu, 1) ? kSmallTile : 0;
+}
+
+int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, const hsgen_kpoint* kps, int nk)
 {
     if (o.tile > 0) return o.tile;
     auto it = ctx->tuned.find(tune_key(o.precision, ng, kdim));
     if (it != ctx->tuned.end()) return it->second;
-    if (o.precision != HSGEN_PREC_FP64) return 0;
-    if (tiles128 <= 0) tiles128 = lower_tiles128(ng);
-    return tiles128 < (int64_t)kSmallWaves * std::max(ctx->n_cu, 1) ? kSmallTile : 0;
+    if (o.precision == HSGEN_PREC_FP64) return fp64_tile_by_size(ctx, ng, kps, nk);
+    if (o.precision == HSGEN_PREC_OZAKI_INT8) {
+        const int64_t t = call_tiles(ng, kps, nk, 256, 192);
+        return t < kInt8FewTiles ? kInt8SmallCfg : t < kInt8SomeTiles ? kInt8MidCfg : 0;
+    }
+    return 0;
 }
 
 }  // namespace hsgen

@@ -246,7 +246,7 @@ def test_bf16_slices_equal_int8_slices_bitwise(ctx, tile):
         Hb, Sb, rb = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, tile=tile, report=True, **kw)
         print(f"tile {tile} {kw}: tiles used {ri['tile_used']} / {rb['tile_used']}, moduli {rb['slices_used']}",
               flush=True)
-        assert rb["tile_used"] == (tile or 61) and ri["tile_used"] == 51
+        assert rb["tile_used"] == (tile or 61) and ri["tile_used"] == 14   # a small call: 128 x 128 INT8 tiles
         assert np.array_equal(Hb, Hi) and np.array_equal(Sb, Si)
         for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
             assert rb[key] == ri[key], key
@@ -410,6 +410,30 @@ def test_batched_split_fixed_moduli_and_flags(ctx):
             assert np.array_equal(H, H1) and np.array_equal(S, S1), (flags, kp.ng)
             assert not (flags & 2 and H.any()) and not (flags & 4 and S.any())
             assert r["slices_used"] == 11
+
+
+@pytest.mark.parametrize("ng,fp64_tile,int8_tile", [(600, 12, 14), (2000, 12, 15), (6002, 11, 51)])
+def test_default_tiles_by_size(ctx, ng, fp64_tile, int8_tile):
+    """Without an explicit or autotuned tile, calls too small to fill the GPU
+    with the large tiles take smaller ones (FP64: 64 x 64 below 6 lower tiles
+    of 128 x 128 per CU; INT8: 128 x 128 below 64 lower tiles of 256 x 192,
+    128 x 256 below 160); every tile gives the same bits."""
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
+    s, kps = make_config(1, ng=ng)
+    kp = kps[0]
+    h = ctx.upload(s)
+    H, S, rep = ctx.create_HS2(h, kp, report=True)
+    H1, S1 = ctx.create_HS2(h, kp, tile=11)
+    print(f"ng {kp.ng}: FP64 tile {rep['tile_used']}", flush=True)
+    assert rep["tile_used"] == fp64_tile
+    assert np.array_equal(H, H1) and np.array_equal(S, S1)
+    Hs, Ss, rs = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=12, report=True)
+    Hs1, Ss1 = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_INT8, slices=12, tile=51)
+    print(f"ng {kp.ng}: INT8 tile {rs['tile_used']}", flush=True)
+    assert rs["tile_used"] == int8_tile
+    assert np.array_equal(Hs, Hs1) and np.array_equal(Ss, Ss1)
+    h.close()
 
 
 def test_autotune_picks_and_applies_a_tile(ctx):
