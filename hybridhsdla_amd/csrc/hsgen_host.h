@@ -42,8 +42,9 @@ constexpr int kDefaultTile = 11;
 constexpr int kSmallTile = 12;
 constexpr int kSmallWaves = 6;
 // INT8 configuration for calls with few S and H lower tiles of 256 x 192 (all
-// k-points): 128 x 128 below kInt8FewTiles, 128 x 256 below kInt8SomeTiles
-constexpr int kInt8SmallCfg = 14, kInt8MidCfg = 15;
+// k-points): 128 x 128 below kInt8FewTiles, 128 x 256 below kInt8SomeTiles;
+// BF16: 256 x 128 below kInt8SomeTiles
+constexpr int kInt8SmallCfg = 14, kInt8MidCfg = 15, kBf16SmallCfg = 62;
 constexpr int kInt8FewTiles = 64, kInt8SomeTiles = 160;
 constexpr int kFallbackTile = 8;   // register-staged, 64-bit addressing
 constexpr int kIgemmCfg = 51;      // default INT8 kernel configuration (256 x 192 tiles)

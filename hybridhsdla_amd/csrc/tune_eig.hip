@@ -105,11 +105,10 @@ int pick_tile(const hsgen_ctx* ctx, const hsgen_opts& o, int ng, int kdim, const
     auto it = ctx->tuned.find(tune_key(o.precision, ng, kdim));
     if (it != ctx->tuned.end()) return it->second;
     if (o.precision == HSGEN_PREC_FP64) return fp64_tile_by_size(ctx, ng, kps, nk);
-    if (o.precision == HSGEN_PREC_OZAKI_INT8) {
-        const int64_t t = call_tiles(ng, kps, nk, 256, 192);
+    const int64_t t = call_tiles(ng, kps, nk, 256, 192);
+    if (o.precision == HSGEN_PREC_OZAKI_INT8)
         return t < kInt8FewTiles ? kInt8SmallCfg : t < kInt8SomeTiles ? kInt8MidCfg : 0;
-    }
-    return 0;
+    return t < kInt8SomeTiles ? kBf16SmallCfg : 0;
 }
 
 }  // namespace hsgen

@@ -232,7 +232,7 @@ def test_split_precision_equals_rounding_model(ctx, monkeypatch, fold):
     assert rel_fro_lower(H, Hm) <= 1e-11
 
 
-@pytest.mark.parametrize("tile", [0, 62, 63])
+@pytest.mark.parametrize("tile", [0, 61, 63])
 def test_bf16_slices_equal_int8_slices_bitwise(ctx, tile):
     """The BF16 variant multiplies the same residues (held as bf16) on BF16
     MFMA with FP32 accumulation, exact below 2^24 and reduced mod p every 32
@@ -246,7 +246,8 @@ def test_bf16_slices_equal_int8_slices_bitwise(ctx, tile):
         Hb, Sb, rb = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, tile=tile, report=True, **kw)
         print(f"tile {tile} {kw}: tiles used {ri['tile_used']} / {rb['tile_used']}, moduli {rb['slices_used']}",
               flush=True)
-        assert rb["tile_used"] == (tile or 61) and ri["tile_used"] == 14   # a small call: 128 x 128 INT8 tiles
+        # a small call: 128 x 128 INT8 tiles, 256 x 128 BF16 tiles by default
+        assert rb["tile_used"] == (tile or 62) and ri["tile_used"] == 14
         assert np.array_equal(Hb, Hi) and np.array_equal(Sb, Si)
         for key in ("slices_used", "err_est_h", "err_est_s", "err_smp_h", "err_smp_s"):
             assert rb[key] == ri[key], key
@@ -412,12 +413,14 @@ def test_batched_split_fixed_moduli_and_flags(ctx):
             assert r["slices_used"] == 11
 
 
-@pytest.mark.parametrize("ng,fp64_tile,int8_tile", [(600, 12, 14), (2000, 12, 15), (6002, 11, 51)])
-def test_default_tiles_by_size(ctx, ng, fp64_tile, int8_tile):
+@pytest.mark.parametrize("ng,fp64_tile,int8_tile,bf16_tile", [(600, 12, 14, 62), (2000, 12, 15, 62),
+                                                                (6002, 11, 51, 61)])
+def test_default_tiles_by_size(ctx, ng, fp64_tile, int8_tile, bf16_tile):
     """Without an explicit or autotuned tile, calls too small to fill the GPU
     with the large tiles take smaller ones (FP64: 64 x 64 below 6 lower tiles
     of 128 x 128 per CU; INT8: 128 x 128 below 64 lower tiles of 256 x 192,
-    128 x 256 below 160); every tile gives the same bits."""
+    128 x 256 below 160; BF16: 256 x 128 below 160); every tile gives the
+    same bits, and BF16 the INT8 bits."""
     from hybridhsdla_amd import make_config
     from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
     s, kps = make_config(1, ng=ng)
@@ -433,6 +436,11 @@ def test_default_tiles_by_size(ctx, ng, fp64_tile, int8_tile):
     print(f"ng {kp.ng}: INT8 tile {rs['tile_used']}", flush=True)
     assert rs["tile_used"] == int8_tile
     assert np.array_equal(Hs, Hs1) and np.array_equal(Ss, Ss1)
+    from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_BF16
+    Hb, Sb, rb = ctx.create_HS2(h, kp, precision=HSGEN_PREC_OZAKI_BF16, slices=12, report=True)
+    print(f"ng {kp.ng}: BF16 tile {rb['tile_used']}", flush=True)
+    assert rb["tile_used"] == bf16_tile
+    assert np.array_equal(Hb, Hs) and np.array_equal(Sb, Ss)
     h.close()
 
 
