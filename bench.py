@@ -424,6 +424,42 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
+def tile_sweep(args) -> int:
+    """Every tile configuration of one precision over problem sizes: the
+    data behind the size-based default tiles (FP64 fixed; split precision at
+    12 moduli)."""
+    import numpy as np
+    import torch
+    from hybridhsdla_amd import make_config
+    from hybridhsdla_amd.api import Context, HSGEN_PREC_FP64, HSGEN_PREC_OZAKI_BF16, HSGEN_PREC_OZAKI_INT8
+
+    prec = {"fp64": HSGEN_PREC_FP64, "int8": HSGEN_PREC_OZAKI_INT8, "bf16": HSGEN_PREC_OZAKI_BF16}[args.precision]
+    tiles = {"fp64": (11, 12, 13, 14), "int8": (51, 11, 12, 13, 14, 15, 16), "bf16": (61, 62, 63)}[args.precision]
+    torch.cuda.init()
+    ctx = Context(0)
+    for cfg, ngs in ((0, [600]), (1, [600, 1200, 2000, 3000, 4000, 6002])):
+        for ng in ngs:
+            s, kps = make_config(cfg, ng=ng)
+            h = ctx.upload(s)
+            kp = kps[0]
+            kw = {} if prec == HSGEN_PREC_FP64 else {"precision": prec, "slices": 12}
+            _, _, rep = ctx.create_HS2(h, kp, report=True, **kw)
+            default = int(rep["tile_used"])
+            out = {}
+            for t in tiles:
+                ms = []
+                for r in range(6):
+                    _, _, rep = ctx.create_HS2(h, kp, tile=t, report=True, **kw)
+                    if r >= 2:
+                        ms.append(rep["ms_total"])
+                out[str(t)] = round(float(np.median(ms)), 4)
+            print(json.dumps({"tile_sweep": args.precision, "workload": f"cfg{cfg}", "n_g": kp.ng, "k_dim": s.kdim,
+                              "ms_total_by_tile": out, "default_tile": default}), flush=True)
+            h.close()
+    ctx.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1, help="number of GPUs; only 1 is measured")
@@ -453,6 +489,10 @@ def main(argv=None) -> int:
                     help="also time one H+S with host (numpy) outputs, i.e. including the PCIe copies")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="GiB: run one device-to-device copy of this size first (traffic calibration)")
+    ap.add_argument("--tile-sweep", action="store_true",
+                    help="instead of the benchmark: whole-call GPU time of every tile of --precision at N_G "
+                         "600..6002 (the 32-atom system) and 600 (the 4-atom one), median of 4, the default "
+                         "tile marked; one JSON line per size")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -467,6 +507,8 @@ def main(argv=None) -> int:
 
     if args.config == 4:
         return bench_batched(args)
+    if args.tile_sweep:
+        return tile_sweep(args)
 
     t0 = time.perf_counter()
     system, kps = make_config(args.config)
