@@ -56,8 +56,11 @@ def test_ragged_sizes_fp64(ctx, big_list, ng):
     sub = _subset(kp, np.arange(ng) * (kp.ng // ng) if ng > 1 else [kp.ng // 2])
     print(f"ng={sub.ng}", flush=True)
     h = ctx.upload(s)
-    H, S = ctx.create_HS2(h, sub)
     Hr, Sr = po.reference_hs(s, sub)
+    # the default tile (64 x 64 at these sizes) and the 128 x 128 tile
+    H, S = ctx.create_HS2(h, sub)
+    H11, S11 = ctx.create_HS2(h, sub, tile=11)
+    assert np.array_equal(H, H11) and np.array_equal(S, S11)
     assert rel_fro_lower(H, Hr) <= TOL and rel_fro_lower(S, Sr) <= TOL
     assert not np.any(np.triu(H, 1)) and not np.any(np.triu(S, 1))
     assert np.all(np.diag(H).imag == 0) and np.all(np.diag(S).imag == 0)
@@ -71,9 +74,12 @@ def test_ragged_sizes_split_precision(ctx, big_list, ng):
     print(f"ng={sub.ng}", flush=True)
     h = ctx.upload(s)
     H, S, rep = ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, report=True)
+    # the 256 x 192 INT8 tile over the same ragged edges: the same bits
+    H51, S51 = ctx.create_HS2(h, sub, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, tile=51)
+    assert np.array_equal(H, H51) and np.array_equal(S, S51)
     Hr, Sr = po.reference_hs(s, sub)
     eh, es = rel_fro_lower(H, Hr), rel_fro_lower(S, Sr)
-    print(f"{rep['slices_used']} moduli: H {eh:.2e} S {es:.2e}", flush=True)
+    print(f"{rep['slices_used']} moduli, tile {rep['tile_used']}: H {eh:.2e} S {es:.2e}", flush=True)
     assert eh <= 1e-12 and es <= 1e-12
     assert not np.any(np.triu(H, 1)) and not np.any(np.triu(S, 1))
 
@@ -214,27 +220,30 @@ def test_folded_and_unfolded_operands_agree(ctx, monkeypatch, big_list):
     assert rel_fro_lower(out["1"][0], out["0"][0]) <= 1e-14 and rel_fro_lower(out["1"][1], out["0"][1]) <= 1e-14
 
 
-def test_batched_ragged_sizes_equal_single_calls(ctx, big_list):
+@pytest.mark.parametrize("tile", [0, 11])
+def test_batched_ragged_sizes_equal_single_calls(ctx, big_list, tile):
     """Host outputs, so that the empty k-point also goes through the
-    device-to-host copy of the batched path."""
+    device-to-host copy of the batched path; the default tile by size and
+    the 128 x 128 tile."""
     s, kp = big_list
     kps = [_subset(kp, [7]), _subset(kp, []), _subset(kp, np.arange(129)), _subset(kp, np.arange(300) * 2),
            _subset(kp, np.arange(2) + 100)]
     h = ctx.upload(s)
-    res, _ = ctx.create_HS2_batched(h, kps)
+    res, _ = ctx.create_HS2_batched(h, kps, tile=tile)
     for (Hb, Sb), k in zip(res, kps):
         H1, S1 = ctx.create_HS2(h, k)
         assert np.array_equal(Hb, H1) and np.array_equal(Sb, S1), k.ng
     assert res[1][0].shape == (0, 0)
 
 
-def test_batched_split_ragged_sizes_equal_single_calls(ctx, big_list):
+@pytest.mark.parametrize("tile", [0, 51])
+def test_batched_split_ragged_sizes_equal_single_calls(ctx, big_list, tile):
     from hybridhsdla_amd.api import HSGEN_PREC_OZAKI_INT8
     s, kp = big_list
     kps = [_subset(kp, [7]), _subset(kp, []), _subset(kp, np.arange(769)), _subset(kp, np.arange(300) * 2),
            _subset(kp, np.arange(2) + 100)]
     h = ctx.upload(s)
-    res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
+    res, reps = ctx.create_HS2_batched(h, kps, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12, tile=tile)
     for (Hb, Sb), k, r in zip(res, kps, reps):
         print(f"ng={k.ng}: {r['slices_used']} moduli", flush=True)
         H1, S1 = ctx.create_HS2(h, k, precision=HSGEN_PREC_OZAKI_INT8, target_rel_err=1e-12)
