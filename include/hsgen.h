@@ -102,7 +102,8 @@ typedef struct {
     double target_rel_err;     /* split precision: target relative Frobenius error (0 = 1e-12) */
     int32_t flags;             /* HSGEN_FILL_UPPER | HSGEN_SKIP_H | HSGEN_SKIP_S */
     int32_t tile;              /* contraction tile config id (0 = the one hsgen_autotune chose
-                                  for this size, else the built-in default) */
+                                  for this size, else the built-in default for the size: smaller
+                                  tiles for calls too small to fill the GPU with the large ones) */
 } hsgen_opts;
 
 typedef struct {
