@@ -770,4 +770,39 @@ hsgen_status hsgen_dump_write(const char* path, const hsgen_system* sys, const i
     return HSGEN_OK;
 }
 
+hsgen_status hsgen_dump_write_hs_packed(const char* path, int32_t k, int32_t n, const double* Hm, const double* Sm,
+                                        int64_t ld)
+{
+    if (!path || !Hm || !Sm || k < 0 || n < 0 || ld < std::max(n, 1)) return fail(HSGEN_ERR_ARG, "bad arguments");
+    quiet();
+    H f(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), H5Fclose);
+    if (!f.ok()) return fail(HSGEN_ERR_ARG, "cannot open %s for writing", path);
+    const std::string nm = std::to_string(k + 1);
+    const hsize_t dims[1] = {(hsize_t)n * (hsize_t)(n + 1) / 2};
+    std::vector<double> buf(2 * (size_t)dims[0]);
+    for (int w = 0; w < 2; ++w) {
+        const char* grp = w == 0 ? "/matrix_hamilton" : "/matrix_overlap";
+        H g(H5Lexists(f, grp, H5P_DEFAULT) > 0 ? H5Gopen2(f, grp, H5P_DEFAULT)
+                                                : H5Gcreate2(f, grp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+            H5Gclose);
+        if (!g.ok()) return fail(HSGEN_ERR_ARG, "cannot create %s in %s", grp, path);
+        if (H5Lexists(g, nm.c_str(), H5P_DEFAULT) > 0)
+            return fail(HSGEN_ERR_ARG, "%s/%s exists in %s", grp, nm.c_str(), path);
+        // the lower triangle row by row: (0,0), (1,0), (1,1), (2,0), ...
+        const double* src = w == 0 ? Hm : Sm;
+        size_t q = 0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j <= i; ++j, ++q) {
+                buf[2 * q] = src[2 * (j * ld + i)];
+                buf[2 * q + 1] = src[2 * (j * ld + i) + 1];
+            }
+        H sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
+        H ct(cx_type(), H5Tclose);
+        H ds(H5Dcreate2(g, nm.c_str(), ct, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        if (!ds.ok() || H5Dwrite(ds, ct, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf.data()) < 0)
+            return fail(HSGEN_ERR_ARG, "cannot write %s/%s in %s", grp, nm.c_str(), path);
+    }
+    return HSGEN_OK;
+}
+
 }  // extern "C"

@@ -44,6 +44,7 @@ IO_SIGNATURES = [
     ("hsgen_unpack_lower", C.c_int, [C.c_int64, _p, _p, C.c_int64]),
     ("hsgen_dump_write", C.c_int, [C.c_char_p, C.POINTER(HsgenSystem), _p, _p, C.c_int32,
                                    C.POINTER(HsgenKpoint)]),
+    ("hsgen_dump_write_hs_packed", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _p, _p, C.c_int64]),
     ("hsgen_store_open", C.c_int, [C.c_char_p, C.POINTER(_p)]),
     ("hsgen_store_add_hs", C.c_int, [_p, C.c_int32, C.c_int32, _p, _p, C.c_int64]),
     ("hsgen_store_close", C.c_int, [_p]),
@@ -234,6 +235,17 @@ def write_dump(path, system: System, kps: Sequence[KPoint]) -> None:
         kst[i].ng = kp.ng
         kst[i].g = g.ctypes.data
     _check(load_io().hsgen_dump_write(str(path).encode(), C.byref(s), _ptr(types), _ptr(amat), len(kps), kst))
+
+
+def write_hs_packed(path, k: int, H: np.ndarray, S: np.ndarray) -> None:
+    """Add reference H and S of k-point k to a dump written by write_dump, in
+    Fleur's packed lower-triangle layout (what FleurDump.fleur_hs reads)."""
+    H = np.asfortranarray(H, dtype=np.complex128)
+    S = np.asfortranarray(S, dtype=np.complex128)
+    n = H.shape[0]
+    if H.shape != (n, n) or S.shape != (n, n):
+        raise ValueError("H and S must be square and of one size")
+    _check(load_io().hsgen_dump_write_hs_packed(str(path).encode(), k, n, _ptr(H), _ptr(S), max(n, 1)))
 
 
 class HSStore:

@@ -95,6 +95,13 @@ hsgen_status hsgen_unpack_lower(int64_t n, const double* packed, double* M, int6
 hsgen_status hsgen_dump_write(const char* path, const hsgen_system* sys, const int32_t* types,
                               const double* amat, int32_t nk, const hsgen_kpoint* kps);
 
+/* Add H and S of k-point k (0-based; datasets "<k+1>") to an existing dump
+ * in Fleur's layout (binmat.f90:194-219): the lower triangles of the n x n
+ * column-major matrices (leading dimension ld) packed row by row, as
+ * hsgen_dump_get_hs_packed reads them.  Gives a synthetic dump the reference
+ * H and S a comparison needs; an existing k-point is an argument error. */
+hsgen_status hsgen_dump_write_hs_packed(const char* path, int32_t k, int32_t n, const double* H, const double* S,
+                                        int64_t ld);
 /* H/S output file.  Replaces HDF5Store (HDF5Store.cpp:6-47). */
 hsgen_status hsgen_store_open(const char* path, hsgen_store** st);
 hsgen_status hsgen_store_add_hs(hsgen_store* st, int32_t k, int32_t n, const double* H,

@@ -161,3 +161,28 @@ def test_hs_store(tmp_path):
     # file row r = matrix column r (the reference writes the column-major buffer)
     M = np.array(nums[0::2]) + 1j * np.array(nums[1::2])
     assert np.array_equal(M.reshape(3, 3), H.T)
+
+
+def test_packed_hs_round_trip(tmp_path):
+    """H and S added to a synthetic dump in Fleur's packed layout read back
+    as Fleur's H and S (lower triangles, bit for bit); writing a k-point twice
+    is an argument error."""
+    s, kps = make_config(0, ng=40)
+    path = tmp_path / "cfg0.h5"
+    require_dump_io()
+    dump_mod.write_dump(path, s, kps + kps)
+    rng = np.random.default_rng(3)
+    mats = []
+    for k in range(2):
+        n = kps[0].ng
+        H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        S = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        dump_mod.write_hs_packed(path, k, H, S)
+        mats.append((H, S))
+    with pytest.raises(RuntimeError):
+        dump_mod.write_hs_packed(path, 1, mats[1][0], mats[1][1])
+    with dump_mod.FleurDump(path) as d:
+        assert d.info["num_hs"] == 2
+        for k, (H, S) in enumerate(mats):
+            Hf, Sf = d.fleur_hs(k)
+            assert np.array_equal(Hf, np.tril(H)) and np.array_equal(Sf, np.tril(S))

@@ -103,3 +103,46 @@ def test_cpp_driver_reproduces_fleur(tmp_path, args, tol):
     assert m, res.stdout
     assert float(m.group(1)) <= tol and float(m.group(2)) <= tol
     assert out.exists() and out.stat().st_size > 2 * 16 * 235 * 235
+
+
+@pytest.fixture(scope="module")
+def multi_k_dump(tmp_path_factory):
+    """The 4-atom workload with three k-points of uneven size written into the
+    dump layout, the oracle's H and S of each added as the reference matrices."""
+    from hybridhsdla_amd import dump as dump_mod, make_config
+    from hybridhsdla_amd.system import KPoint, gvectors, kmax_for_ng
+    from oracle import pyoracle as po
+    s, kps0 = make_config(0, ng=300)
+    b33 = s.bmat33()
+    kps = [kps0[0]]
+    for k, ng in (((0.25, 0.0, 0.1), 220), ((0.5, 0.5, 0.0), 380)):
+        k = np.array(k)
+        kmax = kmax_for_ng(k, b33, ng)
+        kps.append(KPoint(k=k, g=gvectors(k, b33, kmax), kmax=kmax))
+    path = tmp_path_factory.mktemp("multi_k") / "dump.h5"
+    open_dump_or_fail(FIXTURE).close()      # fails where HDF5 is installed but the library is not
+    dump_mod.write_dump(path, s, kps)
+    for i, kp in enumerate(kps):
+        Hr, Sr = po.reference_hs(s, kp)
+        dump_mod.write_hs_packed(path, i, Hr, Sr)
+    return path, [kp.ng for kp in kps]
+
+
+@pytest.mark.parametrize("args,tol", [([], 1e-13), (["--batched"], 1e-13),
+                                      (["--precision", "int8", "--target", "1e-12", "--batched"], 1e-12)])
+def test_cpp_driver_on_a_multi_k_dump(tmp_path, multi_k_dump, args, tol):
+    """The driver over every k-point of a synthetic dump, one call per k-point
+    or one batched call, against the oracle's H and S stored in the dump."""
+    drv = driver_or_fail()
+    path, ngs = multi_k_dump
+    out = tmp_path / "flapw_out.h5"
+    cmd = [str(drv), str(path), "--out", str(out), "--check", str(tol), *args]
+    print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr, flush=True)
+    assert res.returncode == 0, res.stderr
+    errs = re.findall(r"k (\d+): ng (\d+).*rel H ([0-9.e+-]+) S ([0-9.e+-]+)", res.stdout)
+    assert [int(n) for _, n, _, _ in errs] == ngs
+    for _, _, eh, es in errs:
+        assert float(eh) <= tol and float(es) <= tol
+    assert out.exists()
